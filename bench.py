@@ -1,0 +1,162 @@
+"""Throughput benchmark of the SCMF Monte Carlo step on one MI355X.
+
+A step is one pass of the hot path over the whole system: a smart-MC sweep
+(every chain makes N sequential moves), NGP density gridding and the omega
+update.  The workload is the single large system: AB diblock melt, N=64,
+fA=0.3, 4,194,304 chains (268,435,456 beads), 256^3 grid, chiN=25, kappaN=70,
+uint16 counts, fp32 coordinates; synthetic Gaussian-walk initial state.
+
+Prints one JSON line.  `value` is bead-moves per second over the timed steps
+(positions, counts and fields resident in HBM throughout).  `roofline` is
+for the dominant kernel (the sweep), from HIP events on the engine's stream;
+`passes` gives the same for the gridding and omega passes.  `cpu_baseline` is
+the CPU oracle (the restated reference move logic, OpenMP over chains) timed
+on a bounded sample of the same workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(spec, opts, target_s=15.0):
+    """Oracle sweep on the first chains of the workload, all host threads."""
+    import numpy as np
+
+    from new_arch_amd.abi import SystemSpec
+    from oracle import oracle as orc
+
+    N = spec.polymers[0].length
+
+    def sub(chains):
+        p = spec.polymers[0]
+        return SystemSpec(n_types=spec.n_types, xn=spec.xn, L=spec.L, n=spec.n, dtNxi=spec.dtNxi,
+                          nref=spec.nref, polymers=[type(p)(length=N, count=chains,
+                                                            monotype=p.monotype)])
+
+    om = np.zeros(spec.n_types * spec.n_cells, np.float32)
+    chains = 4096
+    o = orc.Oracle(sub(chains), opts)
+    pos = o.init_coords()
+    t = time.perf_counter()
+    o.sweep(pos, om, 0)
+    dt = time.perf_counter() - t
+    per_chain_sweep = dt / chains
+    sweeps = 2
+    chains = int(max(4096, min(spec.n_chains, target_s / (sweeps * per_chain_sweep))))
+    o = orc.Oracle(sub(chains), opts)
+    pos = o.init_coords()
+    t = time.perf_counter()
+    for s in range(sweeps):
+        o.sweep(pos, om, s)
+    dt = time.perf_counter() - t
+    moves = chains * N * sweeps
+    return {"value": moves / dt, "unit": "bead-moves/s", "cores": orc.threads(), "kind": "port",
+            "sample": f"oracle smart-MC sweep of the first {chains} chains ({chains * N} beads) "
+                      f"x {sweeps} sweeps, {dt:.1f} s; gridding and omega not included"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="large_diblock_melt")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this engine runs one system per GPU and measures a single GPU; "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    from new_arch_amd import scmf
+    from new_arch_amd import workloads as W
+    from new_arch_amd.abi import make_opts
+
+    spec = W.NAMED[args.workload]()
+    move = "trial" if args.workload == "small_diblock_melt" else "smart"
+    bits = 32 if args.workload == "largest_blend" else 16
+    opts = make_opts(seed=0, move=move, trial_delta=0.1, omega_every_step=True, density_bits=bits)
+    log(f"creating {args.workload}: {spec.n_beads} beads, {spec.n_chains} chains, "
+        f"grid {spec.n}")
+    t = time.perf_counter()
+    g = scmf.System(spec, opts)
+    g.grid_density()
+    g.update_omega()
+    g.sync()
+    log(f"created in {time.perf_counter() - t:.1f} s")
+    for i in range(args.warmup):
+        g.step(i + 1)
+    g.sync()
+    log(f"warmup {args.warmup} steps done")
+    g.enable_timing(True)
+    g.reset_timing()
+    g.reset_stats()
+    g.sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        g.step(args.warmup + i + 1)
+    g.sync()
+    t1 = time.perf_counter()
+    tm = g.timing()
+    st = g.stats()
+    log(f"timed {args.steps} steps in {t1 - t0:.3f} s; sweep {tm.sweep_ms:.3f} ms, "
+        f"grid {tm.grid_ms:.3f} ms, omega {tm.omega_ms:.3f} ms; acceptance "
+        f"{st.accepts / max(st.moves, 1):.3f}")
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = spec.n_beads * args.steps / (t1 - t0)
+
+    B, TC = spec.n_beads, spec.n_types * spec.n_cells
+    s = 2 if bits == 16 else 4
+    sweep_bytes = 24 * B + 4 * TC
+    grid_bytes = 12 * B + s * TC
+    omega_bytes = s * TC + 4 * TC
+
+    def roof(nbytes, ms):
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": nbytes, "ms": round(ms, 4)}
+
+    out = {
+        "metric": "bead-moves/s/GPU",
+        "value": value,
+        "unit": "bead-moves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 coords / f64 energy+binning / u16 counts" if bits == 16 else
+                 "f32 coords / f64 energy+binning / u32 counts",
+        "data": "synthetic (seeded Gaussian-walk chains, uniform chain starts)",
+        "config": {"workload": args.workload, "beads": B, "chains": spec.n_chains,
+                   "N": [p.length for p in spec.polymers], "grid": list(spec.n),
+                   "box_Re": list(spec.L), "xn": spec.xn, "move": move,
+                   "density_bits": bits, "omega": "every sweep", "parallelism": "1 GPU"},
+        "roofline": dict(roof(sweep_bytes, tm.sweep_ms), kernel="k_sweep"),
+        "passes": {"grid": dict(roof(grid_bytes, tm.grid_ms), kernel="k_zero+k_grid"),
+                   "omega": dict(roof(omega_bytes, tm.omega_ms), kernel="k_omega")},
+        "acceptance": st.accepts / max(st.moves, 1),
+    }
+    g.close()
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(spec, opts)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

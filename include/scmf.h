@@ -1,0 +1,184 @@
+/* scmf.h — C ABI of the MI355X single-chain-in-mean-field Monte Carlo engine.
+ *
+ * The drop-in boundary for the reference's host "operator API" (a set of
+ * extern C++ functions taking Phase, GPU_info and MPI_info pointers).  Every entry point
+ * below names the reference function it replaces.  Conventions:
+ *   - plain C types only; the opaque handle owns all device memory;
+ *   - every call returns SCMF_OK (0) or a negative SCMF_E* code and never
+ *     exits the process; scmf_last_error() gives the message of the last
+ *     failure on the calling thread;
+ *   - work is queued on the handle's own HIP stream; calls that hand data to
+ *     the host (scmf_copy_out, scmf_stats, the writers) synchronise it;
+ *   - one handle is used by one host thread at a time.
+ * Positions cross the boundary as the reference stores them: float
+ * [bead][xyz], chains contiguous, polymer types in polymer.dat order
+ * (struct.h:203, init_config.cu:610-629).
+ */
+#ifndef SCMF_H
+#define SCMF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SCMF_OK 0
+#define SCMF_E_ARG -1      /* invalid argument or inconsistent description */
+#define SCMF_E_IO -2       /* file missing or malformed */
+#define SCMF_E_HIP -3      /* HIP runtime failure */
+#define SCMF_E_NOMEM -4    /* device allocation failed */
+#define SCMF_E_OVERFLOW -5 /* a uint16 density count exceeded 65535 */
+#define SCMF_E_NODEV -6    /* no usable GPU */
+#define SCMF_E_BUDGET -7   /* admission refused: HBM budget exhausted */
+
+/* Move kinds.  SMART is the reference's force-biased move (struct.h:45
+ * move_type_arg_SMART, device_function.cu:637-717).  TRIAL is the uniform
+ * local displacement the reference only names as an enum (struct.h:45). */
+#define SCMF_MOVE_TRIAL 0
+#define SCMF_MOVE_SMART 1
+
+/* What scmf_copy_out / scmf_copy_in move. */
+#define SCMF_POS 0      /* float [beads][3], reference order */
+#define SCMF_DENSITY 1  /* uint32 [types][cells] (widened from uint16 if needed) */
+#define SCMF_OMEGA 2    /* float  [types][cells] */
+
+/* System description: the content of configuration.dat + polymer.dat
+ * (init_config.cu:210-451, 487-514). */
+typedef struct scmf_desc {
+  int32_t n_mono_types;      /* T: "Monomer type number" */
+  const double* xn;          /* [T*T], xn[t + T*s]: diagonal kappaN, off-diagonal chiN */
+  double L[3];               /* box edge lengths in Re: "D0 = ; LY = ; LZ = ;" */
+  int32_t n[3];              /* grid cells per edge: "N = nx ny nz" */
+  double dtNxi;              /* "dt*N/xi"; smart-MC step A = dtNxi / reference_Nbeads */
+  int32_t reference_Nbeads;  /* "Polymer length unit" (Nref) */
+  int32_t n_poly_types;      /* P: "Number of polymer type" */
+  const int32_t* poly_length;  /* [P] beads per chain of each type */
+  const int64_t* n_polymers;   /* [P] chains of each type */
+  const int32_t* monotype;     /* [sum N_p] monomer type of every bead, types concatenated */
+  const uint8_t* connection;   /* [sum N_p*N_p] row-major bond matrices, types concatenated:
+                                  connection[off_p + i*N_p + j] != 0 <=> beads i and j bonded */
+} scmf_desc;
+
+typedef struct scmf_opts {
+  uint64_t seed;            /* Philox key of every random stream of this system */
+  uint32_t system_id;       /* fourth counter word: separates systems sharing a seed */
+  int32_t move_type;        /* SCMF_MOVE_SMART (reference) or SCMF_MOVE_TRIAL */
+  float trial_delta;        /* TRIAL: each component uniform in [-delta, delta) Re */
+  int32_t omega_every_step; /* 0: reference cadence, omega skipped when step%10==0
+                               (test.cpp:42-43); 1: omega after every sweep */
+  int32_t density_bits;     /* 16 (uint16 counts, overflow-checked) or 32 */
+  int32_t init_coords;      /* 1: build the initial Gaussian walks on the device
+                               (initialize_coord, device_function.cu:140-207);
+                               0: the caller supplies positions (scmf_copy_in_pos) */
+} scmf_opts;
+
+typedef struct scmf_stats_t {
+  uint64_t moves;        /* attempted moves since creation / last reset */
+  uint64_t accepts;      /* accepted moves */
+  uint64_t sweeps;       /* sweeps done (also the sweep word of the next sweep's stream) */
+  double re2;            /* mean squared end-to-end distance, first to last bead (Re^2) */
+  int32_t overflow;      /* 1 if a uint16 count saturated since the last gridding */
+} scmf_stats_t;
+
+typedef struct scmf_timing_t {
+  /* Mean device time per launch over the launches since the last reset,
+   * measured with HIP events on the handle's stream. */
+  double sweep_ms, grid_ms, omega_ms;
+  uint64_t sweep_n, grid_n, omega_n;
+} scmf_timing_t;
+
+typedef struct scmf_summary {
+  int32_t n_mono_types, n_poly_types, reference_Nbeads;
+  int32_t n[3];
+  double L[3];
+  double dtNxi;
+  int64_t n_chains, n_beads, n_cells;
+  double xn[64];             /* first min(T*T, 64) entries of xn[t + T*s] */
+  uint64_t device_bytes_16;  /* device memory a system needs with uint16 counts */
+  uint64_t device_bytes_32;  /* ... with uint32 counts */
+} scmf_summary;
+
+typedef struct scmf_sys scmf_sys;
+
+/* init_all_config (init_config.cu:992-1011): validate, derive constants
+ * (init_config.cu:480-484, 662-672, 784-809), allocate device state and,
+ * when opts->init_coords, build the initial configuration. */
+int scmf_create(scmf_sys** out, const scmf_desc* desc, const scmf_opts* opts);
+
+/* init_all_config reading configuration.dat and polymer.dat from the given
+ * paths (Read_polymer_config init_config.cu:210-451, initialize_values
+ * init_config.cu:487-514). */
+int scmf_create_from_files(scmf_sys** out, const char* configuration_dat, const char* polymer_dat,
+                           const scmf_opts* opts);
+
+void scmf_destroy(scmf_sys* sys);
+
+/* smc_move (smc.cu:3-72 -> mc_polymer_move_arr, device_function.cu:718-799):
+ * n_sweeps sweeps, each giving every chain N sequential moves against the
+ * current omega field. */
+int scmf_mc_sweep(scmf_sys* sys, int32_t n_sweeps);
+
+/* density_reset_mpi / reset_density_fields (mesh_mpi.cpp:4-27, mesh.cu:161-190
+ * -> init_array + coord_to_density, device_function.cu:471-508): zero and
+ * re-grid the NGP density counts.  The multi-GPU reduction collapses to
+ * nothing on one device. */
+int scmf_grid_density(scmf_sys* sys);
+
+/* update_omega_fields_scmf0 (mesh.cu:217-248 -> omega_field_update,
+ * device_function.cu:373-412). */
+int scmf_update_omega(scmf_sys* sys);
+
+/* One iteration of the step loop body of test.cpp:37-43: sweep, grid, then
+ * omega unless the reference cadence skips it for this step number. */
+int scmf_step(scmf_sys* sys, int32_t step);
+
+/* Replace the host's direct reads of managed memory (io.cpp:36,44,107,
+ * test.cpp:51-58): copy a device array to host memory. */
+int scmf_copy_out(scmf_sys* sys, int32_t what, void* host, size_t bytes);
+
+/* coord.dat restart (init_config.cu:930-987) and parity injection. */
+int scmf_copy_in_pos(scmf_sys* sys, const float* xyz, size_t n_beads);
+int scmf_copy_in_omega(scmf_sys* sys, const float* omega, size_t n_values);
+/* Density counts [types][cells] (uint32; must fit the system's count width). */
+int scmf_copy_in_density(scmf_sys* sys, const uint32_t* counts, size_t n_values);
+
+/* Re-key the random streams: seed, and the sweep counter of the next sweep. */
+int scmf_set_rng(scmf_sys* sys, uint64_t seed, uint64_t next_sweep);
+
+int scmf_stats(scmf_sys* sys, scmf_stats_t* out);
+int scmf_reset_stats(scmf_sys* sys);
+/* Per-kernel device timing with HIP events (off by default; the reference
+ * times one whole step with cudaEvents, test.cpp:82-95). */
+int scmf_enable_timing(scmf_sys* sys, int32_t on);
+int scmf_timing(scmf_sys* sys, scmf_timing_t* out);
+int scmf_reset_timing(scmf_sys* sys);
+int scmf_sync(scmf_sys* sys);
+
+/* Sizes of the system (for host buffers). */
+int scmf_sizes(scmf_sys* sys, int64_t* n_beads, int64_t* n_chains, int64_t* n_cells,
+               int32_t* n_types);
+
+/* io_out_density (io.cpp:4-56): density_<step>.dat in directory dir ("" = cwd). */
+int scmf_write_density(scmf_sys* sys, const char* dir, int32_t step);
+/* io_out_coord (io.cpp:95-118): "%g %g %g" per bead. */
+int scmf_write_coord(scmf_sys* sys, const char* path);
+/* Host-only helpers (no device needed). */
+/* Parse configuration.dat + polymer.dat and report sizes and memory needs. */
+int scmf_inspect_files(const char* configuration_dat, const char* polymer_dat, scmf_summary* out);
+/* Sizes and memory needs of a description. */
+int scmf_inspect_desc(const scmf_desc* desc, scmf_summary* out);
+/* io_out_density's format from host arrays: counts uint32 [T][cells], omega float [T][cells]. */
+int scmf_write_density_host(const char* path, const int32_t n[3], const double L[3], int32_t T,
+                            const uint32_t* counts, const float* omega, double field_scaling);
+/* vtk.c:4-81: density_*.dat -> STRUCTURED_POINTS VTK of phi_0 - phi_1. */
+int scmf_density_to_vtk(const char* density_path, const char* vtk_path);
+
+const char* scmf_last_error(void);
+const char* scmf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SCMF_H */

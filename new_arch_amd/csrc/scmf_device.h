@@ -1,0 +1,89 @@
+// scmf_device.h — device-side system description and the per-bead arithmetic
+// of the SCMF Monte Carlo path, written for gfx950 (wave64).
+//
+// The arithmetic mirrors the reference operation for operation (cited per
+// function) so results are bit-identical to the CPU oracle; the data layout is
+// this engine's own: planar fp32 coordinates x[], y[], z[] with every chain's
+// beads contiguous, counts as [type][z][y][x].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "scmf_prims.h"
+
+#pragma clang fp contract(off)
+
+namespace scmf {
+
+constexpr int MAX_T = 8;   // monomer types
+constexpr int MAX_P = 16;  // polymer architectures
+
+// Everything a kernel needs about one system, passed by value.
+struct KParams {
+  int T, P;
+  int nx, ny, nz;
+  uint32_t n_cells;
+  double Lx, Ly, Lz, dx, dy, dz;
+  double harm;                   // harmonic_normb
+  double inv_nref;               // inverse_refbeads
+  double fst[MAX_T];             // field_scaling_type
+  double xn[MAX_T * MAX_T];      // [t + T*s]
+  float A[MAX_T], R[MAX_T];      // smart-MC step, as the move reads them (fp32)
+  int64_t chain_base[MAX_P + 1]; // polymer_basis_gpu
+  int64_t bead_base[MAX_P + 1];  // monomer_poly_basis_gpu
+  int len[MAX_P];
+  int mono_off[MAX_P + 1];
+  const int* monotype;           // [sum N_p]
+  const int* nbr_start;          // [sum N_p + 1], CSR over every bead of every type
+  const int* nbr;                // neighbour bead indices within the chain
+  int64_t n_chains, n_beads;
+  uint64_t seed;
+  uint32_t system_id;
+  int move_type;
+  float trial_delta;
+};
+
+// coord_to_index_unified_high, device_function.cu:313-360 (one axis).
+// Coordinates the reference cannot bin (non-finite, or beyond 2^31 box
+// lengths, where its int conversions overflow) land in cell 0.
+__device__ __forceinline__ int fold_axis(double r, double L, double d, int n) {
+  int i = 0;
+  if (r >= 0) {
+    const double q = r / L;
+    if (!(q < 2147483647.0)) return 0;
+    const int k = (int)q;
+    i = (int)((r - L * k) / d);
+    i = i % n;
+  } else if (r < 0) {
+    const double q = -r / L;
+    if (!(q < 2147483646.0)) return 0;
+    const int k = (int)q + 1;
+    i = (int)((k * L + r) / d);
+    i = i % n;
+  }
+  if ((unsigned)i >= (unsigned)n) i = 0;
+  return i;
+}
+
+__device__ __forceinline__ uint32_t bin_cell(const KParams& kp, double rx, double ry, double rz,
+                                             uint32_t type) {
+  const int xi = fold_axis(rx, kp.Lx, kp.dx, kp.nx);
+  const int yi = fold_axis(ry, kp.Ly, kp.dy, kp.ny);
+  const int zi = fold_axis(rz, kp.Lz, kp.dz, kp.nz);
+  return type * kp.n_cells + (uint32_t)(xi + yi * kp.nx + zi * kp.nx * kp.ny);
+}
+
+// polymer_number_to_index (device_function.cu:72-93): architecture of a chain.
+__device__ __forceinline__ int arch_of_chain(const KParams& kp, int64_t chain) {
+  int p = 0;
+  while (p < kp.P - 1 && chain >= kp.chain_base[p + 1]) ++p;
+  return p;
+}
+
+__device__ __forceinline__ int arch_of_bead(const KParams& kp, int64_t bead) {
+  int p = 0;
+  while (p < kp.P - 1 && bead >= kp.bead_base[p + 1]) ++p;
+  return p;
+}
+
+}  // namespace scmf

@@ -1,0 +1,245 @@
+// scmf_prims.h — arithmetic primitives shared, bit for bit, by the HIP kernels
+// and the CPU oracle.
+//
+// What lives here is the *definition* of the injected random streams and of the
+// special functions the Monte Carlo path needs:
+//   * Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel Random Numbers: As
+//     Easy as 1, 2, 3", SC'11), keyed by the run seed, counter-addressed by
+//     (chain, sweep, move, slot, system);
+//   * uniform transforms of its 32-bit outputs;
+//   * Box–Muller normals (Box & Muller 1958) built from our own log and sincos;
+//   * our own exp (Cody & Waite 1980 argument reduction + Taylor polynomial).
+// The reference draws from cuRAND (curand_uniform, curand_normal4,
+// curand_uniform_double, curand_normal4_double; device_function.cu:163-183,
+// 683, 749) and calls CUDA's exp (device_function.cu:447).  Neither can be
+// reproduced here, so both sides consume these streams instead.
+//
+// Every operation below is a single IEEE-754 operation (+ - * / sqrt fma,
+// conversions, floor) so g++ (x86-64 SSE2) and hipcc (gfx950) produce the same
+// bits.  Both compilers must build this with FP contraction off; fma is only
+// ever requested explicitly.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define SCMF_HD __host__ __device__ __forceinline__
+#else
+#define SCMF_HD static inline
+#endif
+
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+
+namespace scmf {
+
+// ---------------------------------------------------------------- Philox4x32-10
+struct u32x4 {
+  uint32_t v[4];
+};
+
+SCMF_HD void philox_round(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
+                          uint32_t k1) {
+  const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+  const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+  const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+  const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+  const uint32_t n0 = hi1 ^ c1 ^ k0;
+  const uint32_t n2 = hi0 ^ c3 ^ k1;
+  c0 = n0;
+  c1 = lo1;
+  c2 = n2;
+  c3 = lo0;
+}
+
+// Ten rounds with the Weyl key schedule (0x9E3779B9, 0xBB67AE85).
+SCMF_HD u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                            uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    philox_round(c0, c1, c2, c3, k0, k1);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  u32x4 out;
+  out.v[0] = c0;
+  out.v[1] = c1;
+  out.v[2] = c2;
+  out.v[3] = c3;
+  return out;
+}
+
+// Stream slots.  Counter layout: {chain, sweep, (index << 2) | slot, system}.
+enum : uint32_t {
+  SLOT_PICK = 0,    // bead pick (53-bit double) + accept uniform (24-bit float)
+  SLOT_DISP = 1,    // displacement: three normals (smart MC) or three uniforms (trial)
+  SLOT_INIT = 2,    // initial configuration (start point, then one block per placed bead)
+  INIT_SWEEP = 0xFFFFFFFFu  // sweep word reserved for the initial configuration
+};
+
+SCMF_HD u32x4 stream_block(uint64_t seed, uint32_t system, uint32_t chain, uint32_t sweep,
+                           uint32_t index, uint32_t slot) {
+  return philox4x32_10(chain, sweep, (index << 2) | slot, system, (uint32_t)seed,
+                       (uint32_t)(seed >> 32));
+}
+
+// ---------------------------------------------------------------- uniforms
+// (0, 1] with 53 random bits: ((hi27 << 26) | lo26) + 1, times 2^-53.  Exact.
+SCMF_HD double u01_53(uint32_t a, uint32_t b) {
+  const uint64_t m = ((uint64_t)(a >> 5) << 26) | (uint64_t)(b >> 6);
+  return (double)(m + 1) * 1.1102230246251565404e-16;  // 2^-53
+}
+// (0, 1] float with 24 random bits.  Exact.
+SCMF_HD float u01_24f(uint32_t a) { return (float)((a >> 8) + 1u) * 5.9604644775390625e-8f; }
+// [0, 1) float with 24 random bits.  Exact.
+SCMF_HD float u01_24f_closed0(uint32_t a) { return (float)(a >> 8) * 5.9604644775390625e-8f; }
+// (0, 1) double from 32 bits: (a + 1/2) 2^-32.  Exact.
+SCMF_HD double u01_32_open(uint32_t a) {
+  return ((double)a + 0.5) * 2.3283064365386962890625e-10;
+}
+
+// ---------------------------------------------------------------- log
+// Natural log for finite x > 0 (normal range).  x = 2^e m, m in [sqrt(1/2),
+// sqrt(2)); log m = 2 atanh(s), s = (m-1)/(m+1), summed as the odd Taylor
+// series of atanh to s^23 (|s| <= 0.1716, truncation < 1e-19).  The split of
+// ln 2 into a 29-bit head and a tail keeps e*LN2_HI exact for |e| < 2^24.
+SCMF_HD double log_pos(double x) {
+  union {
+    double d;
+    uint64_t u;
+  } bits;
+  bits.d = x;
+  int e = (int)((bits.u >> 52) & 0x7FF) - 1023;
+  bits.u = (bits.u & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull;  // m in [1, 2)
+  double m = bits.d;
+  if (m > 1.4142135623730951) {
+    m = m * 0.5;
+    e += 1;
+  }
+  const double f = m - 1.0;
+  const double s = f / (m + 1.0);
+  const double z = s * s;
+  // atanh(s)/s = sum_k z^k / (2k+1), k = 0..11, Horner from the top.
+  double p = 1.0 / 23.0;
+  p = __builtin_fma(p, z, 1.0 / 21.0);
+  p = __builtin_fma(p, z, 1.0 / 19.0);
+  p = __builtin_fma(p, z, 1.0 / 17.0);
+  p = __builtin_fma(p, z, 1.0 / 15.0);
+  p = __builtin_fma(p, z, 1.0 / 13.0);
+  p = __builtin_fma(p, z, 1.0 / 11.0);
+  p = __builtin_fma(p, z, 1.0 / 9.0);
+  p = __builtin_fma(p, z, 1.0 / 7.0);
+  p = __builtin_fma(p, z, 1.0 / 5.0);
+  p = __builtin_fma(p, z, 1.0 / 3.0);
+  // log m = 2 s + 2 s z p'  where p' is the series without its leading 1.
+  const double two_s = 2.0 * s;
+  const double logm = __builtin_fma(two_s * z, p, two_s);
+  const double LN2_HI = 0x1.62e42fep-1;           // ln 2 with its low 24 bits cleared
+  const double LN2_LO = 0x1.f473de6af278fp-30;    // ln 2 - LN2_HI
+  const double de = (double)e;
+  return __builtin_fma(de, LN2_HI, __builtin_fma(de, LN2_LO, logm));
+}
+
+// ---------------------------------------------------------------- sincos(2 pi u)
+// For u in [0, 1): quarter-turn reduction is exact because u is a multiple of
+// 2^-32; the remaining angle t = 2 pi v, |v| <= 1/8, goes through the Taylor
+// series of sin (to t^19) and cos (to t^18).
+SCMF_HD void sincos_2pi(double u, double* s_out, double* c_out) {
+  const double q = __builtin_floor(u * 4.0 + 0.5);  // nearest quarter turn, 0..4
+  const double v = u - q * 0.25;                     // exact, |v| <= 1/8
+  const double t = v * 6.283185307179586232;         // 2 pi
+  const double t2 = t * t;
+  double sp = -1.0 / 121645100408832000.0;           // -1/19!
+  sp = __builtin_fma(sp, t2, 1.0 / 355687428096000.0);
+  sp = __builtin_fma(sp, t2, -1.0 / 1307674368000.0);
+  sp = __builtin_fma(sp, t2, 1.0 / 6227020800.0);
+  sp = __builtin_fma(sp, t2, -1.0 / 39916800.0);
+  sp = __builtin_fma(sp, t2, 1.0 / 362880.0);
+  sp = __builtin_fma(sp, t2, -1.0 / 5040.0);
+  sp = __builtin_fma(sp, t2, 1.0 / 120.0);
+  sp = __builtin_fma(sp, t2, -1.0 / 6.0);
+  const double sn = __builtin_fma(t * t2, sp, t);
+  double cp = 1.0 / 6402373705728000.0;              // 1/18!
+  cp = __builtin_fma(cp, t2, -1.0 / 20922789888000.0);
+  cp = __builtin_fma(cp, t2, 1.0 / 87178291200.0);
+  cp = __builtin_fma(cp, t2, -1.0 / 479001600.0);
+  cp = __builtin_fma(cp, t2, 1.0 / 3628800.0);
+  cp = __builtin_fma(cp, t2, -1.0 / 40320.0);
+  cp = __builtin_fma(cp, t2, 1.0 / 720.0);
+  cp = __builtin_fma(cp, t2, -1.0 / 24.0);
+  cp = __builtin_fma(cp, t2, 0.5);
+  const double cs = __builtin_fma(-t2, cp, 1.0);
+  const int quad = ((int)q) & 3;
+  double s, c;
+  if (quad == 0) {
+    s = sn;
+    c = cs;
+  } else if (quad == 1) {
+    s = cs;
+    c = -sn;
+  } else if (quad == 2) {
+    s = -sn;
+    c = -cs;
+  } else {
+    s = -cs;
+    c = sn;
+  }
+  *s_out = s;
+  *c_out = c;
+}
+
+// ---------------------------------------------------------------- normals
+// Three standard normals from one Philox block: (w0, w1) -> Box–Muller pair,
+// (w2, w3) -> the cosine branch only (the fourth normal is not needed).
+SCMF_HD void normal3(const u32x4& w, double* n0, double* n1, double* n2) {
+  const double r0 = __builtin_sqrt(-2.0 * log_pos(u01_32_open(w.v[0])));
+  double s0, c0;
+  sincos_2pi((double)w.v[1] * 2.3283064365386962890625e-10, &s0, &c0);
+  const double r1 = __builtin_sqrt(-2.0 * log_pos(u01_32_open(w.v[2])));
+  double s1, c1;
+  sincos_2pi((double)w.v[3] * 2.3283064365386962890625e-10, &s1, &c1);
+  *n0 = r0 * c0;
+  *n1 = r0 * s0;
+  *n2 = r1 * c1;
+}
+
+// ---------------------------------------------------------------- exp
+// exp(x) in double.  k = nearest integer to x/ln2; r = x - k ln2 in two steps
+// (Cody & Waite 1980), |r| <= 0.347; exp(r) by its Taylor series to r^14
+// (truncation < 2e-18 relative); scale by 2^k in two halves so gradual
+// underflow is handled.  Overflow returns +inf, deep underflow returns 0.
+SCMF_HD double exp_cw(double x) {
+  if (x > 709.782712893384) return __builtin_inf();
+  if (x < -745.2) return 0.0;
+  if (!(x == x)) return x;
+  const double kd = __builtin_floor(x * 1.4426950408889634074 + 0.5);
+  const double LN2_HI = 0x1.62e42fep-1;
+  const double LN2_LO = 0x1.f473de6af278fp-30;
+  const double r = __builtin_fma(-kd, LN2_LO, __builtin_fma(-kd, LN2_HI, x));
+  double p = 1.0 / 87178291200.0;  // 1/14!
+  p = __builtin_fma(p, r, 1.0 / 6227020800.0);
+  p = __builtin_fma(p, r, 1.0 / 479001600.0);
+  p = __builtin_fma(p, r, 1.0 / 39916800.0);
+  p = __builtin_fma(p, r, 1.0 / 3628800.0);
+  p = __builtin_fma(p, r, 1.0 / 362880.0);
+  p = __builtin_fma(p, r, 1.0 / 40320.0);
+  p = __builtin_fma(p, r, 1.0 / 5040.0);
+  p = __builtin_fma(p, r, 1.0 / 720.0);
+  p = __builtin_fma(p, r, 1.0 / 120.0);
+  p = __builtin_fma(p, r, 1.0 / 24.0);
+  p = __builtin_fma(p, r, 1.0 / 6.0);
+  p = __builtin_fma(p, r, 0.5);
+  p = __builtin_fma(p, r, 1.0);
+  const double er = __builtin_fma(p, r, 1.0);
+  const int k = (int)kd;
+  const int k1 = k / 2;
+  const int k2 = k - k1;
+  union {
+    double d;
+    uint64_t u;
+  } s1, s2;
+  s1.u = (uint64_t)(k1 + 1023) << 52;
+  s2.u = (uint64_t)(k2 + 1023) << 52;
+  return (er * s1.d) * s2.d;
+}
+
+}  // namespace scmf

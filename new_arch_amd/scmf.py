@@ -1,0 +1,201 @@
+"""Python binding of libscmf.so (the HIP engine) through its C ABI.
+
+The engine must be built (new_arch_amd/build.py); importing this module on a
+machine without the library raises instead of falling back to anything else.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import (DENSITY, OMEGA, POS, SCMF_OK, Desc, Opts, Stats, Summary, SystemSpec,
+                  Timing, make_opts)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libscmf.so")
+
+_lib = None
+
+
+class ScmfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"scmf error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load libscmf.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with new_arch_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.scmf_create.argtypes = [C.POINTER(vp), C.POINTER(Desc), C.POINTER(Opts)]
+    L.scmf_create_from_files.argtypes = [C.POINTER(vp), C.c_char_p, C.c_char_p, C.POINTER(Opts)]
+    L.scmf_destroy.argtypes = [vp]
+    L.scmf_destroy.restype = None
+    for name in ("scmf_mc_sweep", "scmf_step", "scmf_enable_timing"):
+        getattr(L, name).argtypes = [vp, C.c_int32]
+    for name in ("scmf_grid_density", "scmf_update_omega", "scmf_reset_stats",
+                 "scmf_reset_timing", "scmf_sync"):
+        getattr(L, name).argtypes = [vp]
+    L.scmf_copy_out.argtypes = [vp, C.c_int32, vp, C.c_size_t]
+    L.scmf_copy_in_pos.argtypes = [vp, vp, C.c_size_t]
+    L.scmf_copy_in_omega.argtypes = [vp, vp, C.c_size_t]
+    L.scmf_copy_in_density.argtypes = [vp, vp, C.c_size_t]
+    L.scmf_set_rng.argtypes = [vp, C.c_uint64, C.c_uint64]
+    L.scmf_stats.argtypes = [vp, C.POINTER(Stats)]
+    L.scmf_timing.argtypes = [vp, C.POINTER(Timing)]
+    L.scmf_sizes.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                             C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+    L.scmf_write_density.argtypes = [vp, C.c_char_p, C.c_int32]
+    L.scmf_write_coord.argtypes = [vp, C.c_char_p]
+    L.scmf_density_to_vtk.argtypes = [C.c_char_p, C.c_char_p]
+    L.scmf_inspect_files.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Summary)]
+    L.scmf_inspect_desc.argtypes = [C.POINTER(Desc), C.POINTER(Summary)]
+    L.scmf_write_density_host.argtypes = [C.c_char_p, C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_double), C.c_int32, vp, vp, C.c_double]
+    L.scmf_last_error.restype = C.c_char_p
+    L.scmf_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != SCMF_OK:
+        raise ScmfError(rc, lib().scmf_last_error().decode())
+
+
+def inspect_files(configuration_dat, polymer_dat) -> Summary:
+    s = Summary()
+    _check(lib().scmf_inspect_files(str(configuration_dat).encode(), str(polymer_dat).encode(),
+                                    C.byref(s)))
+    return s
+
+
+def inspect_spec(spec: SystemSpec) -> Summary:
+    d, keep = spec.to_desc()
+    s = Summary()
+    _check(lib().scmf_inspect_desc(C.byref(d), C.byref(s)))
+    return s
+
+
+def density_to_vtk(density_path, vtk_path):
+    _check(lib().scmf_density_to_vtk(str(density_path).encode(), str(vtk_path).encode()))
+
+
+class System:
+    """One SCMF system on the current GPU, driven through the C ABI."""
+
+    def __init__(self, spec: SystemSpec = None, opts: Opts = None, *, files=None, **kw):
+        L = lib()
+        self._h = C.c_void_p()
+        if opts is None:
+            opts = make_opts(**kw)
+        self.opts = opts
+        if files is not None:
+            cfg, poly = files
+            _check(L.scmf_create_from_files(C.byref(self._h), str(cfg).encode(),
+                                            str(poly).encode(), C.byref(opts)))
+        else:
+            d, self._keep = spec.to_desc()
+            _check(L.scmf_create(C.byref(self._h), C.byref(d), C.byref(opts)))
+        self.spec = spec
+        nb, nc, ncell, nt = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        _check(L.scmf_sizes(self._h, C.byref(nb), C.byref(nc), C.byref(ncell), C.byref(nt)))
+        self.n_beads, self.n_chains, self.n_cells, self.n_types = (nb.value, nc.value,
+                                                                   ncell.value, nt.value)
+
+    # -- lifetime
+    def close(self):
+        if self._h:
+            lib().scmf_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- the reference call sequence
+    def mc_sweep(self, n=1):
+        _check(lib().scmf_mc_sweep(self._h, int(n)))
+
+    def grid_density(self):
+        _check(lib().scmf_grid_density(self._h))
+
+    def update_omega(self):
+        _check(lib().scmf_update_omega(self._h))
+
+    def step(self, step):
+        _check(lib().scmf_step(self._h, int(step)))
+
+    def sync(self):
+        _check(lib().scmf_sync(self._h))
+
+    # -- data movement
+    def positions(self) -> np.ndarray:
+        out = np.empty((self.n_beads, 3), np.float32)
+        _check(lib().scmf_copy_out(self._h, POS, out.ctypes.data, out.nbytes))
+        return out
+
+    def density(self) -> np.ndarray:
+        out = np.empty(self.n_types * self.n_cells, np.uint32)
+        _check(lib().scmf_copy_out(self._h, DENSITY, out.ctypes.data, out.nbytes))
+        return out
+
+    def omega(self) -> np.ndarray:
+        out = np.empty(self.n_types * self.n_cells, np.float32)
+        _check(lib().scmf_copy_out(self._h, OMEGA, out.ctypes.data, out.nbytes))
+        return out
+
+    def set_positions(self, xyz):
+        a = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        _check(lib().scmf_copy_in_pos(self._h, a.ctypes.data, a.shape[0]))
+
+    def set_omega(self, om):
+        a = np.ascontiguousarray(om, np.float32).ravel()
+        _check(lib().scmf_copy_in_omega(self._h, a.ctypes.data, a.size))
+
+    def set_density(self, counts):
+        a = np.ascontiguousarray(counts, np.uint32).ravel()
+        _check(lib().scmf_copy_in_density(self._h, a.ctypes.data, a.size))
+
+    def set_rng(self, seed, next_sweep):
+        _check(lib().scmf_set_rng(self._h, int(seed), int(next_sweep)))
+
+    # -- diagnostics
+    def stats(self) -> Stats:
+        s = Stats()
+        _check(lib().scmf_stats(self._h, C.byref(s)))
+        return s
+
+    def reset_stats(self):
+        _check(lib().scmf_reset_stats(self._h))
+
+    def enable_timing(self, on=True):
+        _check(lib().scmf_enable_timing(self._h, 1 if on else 0))
+
+    def timing(self) -> Timing:
+        t = Timing()
+        _check(lib().scmf_timing(self._h, C.byref(t)))
+        return t
+
+    def reset_timing(self):
+        _check(lib().scmf_reset_timing(self._h))
+
+    # -- files
+    def write_density(self, step, directory=""):
+        _check(lib().scmf_write_density(self._h, str(directory).encode(), int(step)))
+
+    def write_coord(self, path):
+        _check(lib().scmf_write_coord(self._h, str(path).encode()))

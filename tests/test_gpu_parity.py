@@ -1,0 +1,267 @@
+"""Parity of the HIP engine (through its C ABI) with the CPU oracle.
+
+Bar: bit-exact positions, counts and accept decisions under the same injected
+Philox streams; omega bit-exact against the oracle's fp64 value rounded once
+to fp32 (hence within 2^-24 relative of the fp64 reference value, inside the
+1e-6 relative tolerance).  Sizes are small enough for the oracle to finish in
+seconds; the full-size systems are checked through size-independent
+invariants.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from new_arch_amd import scmf
+from new_arch_amd import workloads as W
+from new_arch_amd.abi import PolymerType, SystemSpec, make_opts
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SPECS = {
+    "tiny": W.tiny,
+    "mixed": W.mixed_arch,
+    "shipped": W.reference_shipped,
+}
+
+
+def banner(msg):
+    print(f"[gpu-parity] {msg}", flush=True)
+
+
+def pair(spec, **kw):
+    o = make_opts(**kw)
+    return scmf.System(spec, o), orc.Oracle(spec, o)
+
+
+def same_bits(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    assert a.shape == b.shape and a.dtype == b.dtype
+    return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+def test_init_bit_exact(name):
+    banner(f"init {name}")
+    spec = SPECS[name]()
+    g, o = pair(spec, seed=12345)
+    assert same_bits(g.positions(), o.init_coords())
+    g.close()
+
+
+@pytest.mark.parametrize("bits", [16, 32])
+@pytest.mark.parametrize("name", list(SPECS))
+def test_grid_bit_exact(name, bits):
+    banner(f"grid {name} u{bits}")
+    spec = SPECS[name]()
+    g, o = pair(spec, seed=7, density_bits=bits)
+    pos = o.init_coords()
+    g.grid_density()
+    np.testing.assert_array_equal(g.density(), o.grid(pos))
+    # unwrapped, far-away and negative coordinates, cell edges
+    rng = np.random.default_rng(3)
+    L = np.array(spec.L)
+    far = (rng.uniform(-40, 40, (spec.n_beads, 3)) * L).astype(np.float32)
+    edges = (rng.integers(-3 * spec.n[0], 3 * spec.n[0], (spec.n_beads // 4, 3))
+             * (L / np.array(spec.n))).astype(np.float32)
+    far[: edges.shape[0]] = edges
+    g.set_positions(far)
+    g.grid_density()
+    np.testing.assert_array_equal(g.density(), o.grid(far))
+    assert g.stats().overflow == 0
+    g.close()
+
+
+@pytest.mark.parametrize("name", list(SPECS))
+def test_omega_bit_exact(name):
+    banner(f"omega {name}")
+    spec = SPECS[name]()
+    g, o = pair(spec, seed=9)
+    g.grid_density()
+    g.update_omega()
+    counts = g.density()
+    ref64 = o.omega(counts)
+    got = g.omega()
+    assert same_bits(got, ref64.astype(np.float32))
+    rel = np.abs(got.astype(np.float64) - ref64) / np.maximum(np.abs(ref64), 1e-300)
+    assert rel[ref64 != 0].max() <= 2.0 ** -24
+    g.close()
+
+
+@pytest.mark.parametrize("step", [240000, 999000])
+def test_omega_reference_dumps_on_gpu(golden, step):
+    banner(f"omega dump {step}")
+    f = np.load(os.path.join(golden, "density_fixtures.npz"))
+    grid = [int(v) for v in f[f"grid_{step}"]]
+    kappa, chi = f[f"kappa_chi_{step}"]
+    poly = PolymerType(length=32, count=12800, monotype=[0] * 4 + [1] * 24 + [2] * 4)
+    spec = SystemSpec(n_types=3, xn=W.xn_matrix(3, kappa, chi), L=[v / 6.0 for v in grid],
+                      n=grid, dtNxi=0.17, nref=32, polymers=[poly])
+    g = scmf.System(spec, make_opts(seed=1, density_bits=16, init_coords=False))
+    prev = np.ascontiguousarray(f[f"prev_{step}"].astype(np.uint32).T)
+    g.set_density(prev)
+    g.update_omega()
+    got = g.omega().reshape(3, -1).T.astype(np.float64)
+    printed = f[f"omega_{step}"].astype(np.float64)
+    assert np.all(np.abs(got - printed) <= 6e-6 * np.abs(printed) + 2e-6)
+    g.close()
+
+
+@pytest.mark.parametrize("move", ["smart", "trial"])
+@pytest.mark.parametrize("name", list(SPECS))
+def test_sweep_bit_exact(name, move):
+    banner(f"sweep {name} {move}")
+    spec = SPECS[name]()
+    g, o = pair(spec, seed=2024, move=move, trial_delta=0.1)
+    pos = o.init_coords()
+    g.grid_density()
+    g.update_omega()
+    om = g.omega()
+    acc_ref = 0
+    for s in range(3):
+        g.mc_sweep(1)
+        acc_ref += o.sweep(pos, om, s)
+        assert same_bits(g.positions(), pos), f"sweep {s}"
+    st = g.stats()
+    assert st.accepts == acc_ref and st.moves == 3 * spec.n_beads
+    assert 0 < st.accepts < st.moves
+    g.close()
+
+
+@pytest.mark.parametrize("cadence", [0, 1])
+def test_step_loop_bit_exact(cadence):
+    """The test.cpp:37-43 loop: sweep, grid, omega unless step%10==0."""
+    banner(f"step loop cadence={cadence}")
+    spec = W.tiny(n_chains=256, N=24, grid=(10, 10, 10), L=(2.5, 2.5, 2.5), chiN=30.0)
+    g, o = pair(spec, seed=77, omega_every_step=bool(cadence), density_bits=16)
+    pos = o.init_coords()
+    g.grid_density()
+    g.update_omega()
+    om = o.omega(o.grid(pos)).astype(np.float32)
+    for step in range(1, 26):
+        g.step(step)
+        o.sweep(pos, om, step - 1)
+        counts = o.grid(pos)
+        if cadence or step % 10:
+            om = o.omega(counts).astype(np.float32)
+    assert same_bits(g.positions(), pos)
+    np.testing.assert_array_equal(g.density(), counts)
+    assert same_bits(g.omega(), om)
+    g.close()
+
+
+def test_small_diblock_melt_two_steps():
+    """The 262,144-bead trial-move melt, omega every sweep, uint16 counts."""
+    banner("small diblock melt")
+    spec = W.small_diblock_melt()
+    g, o = pair(spec, seed=0, move="trial", trial_delta=0.1, omega_every_step=True,
+                density_bits=16)
+    pos = o.init_coords()
+    g.grid_density()
+    g.update_omega()
+    om = o.omega(o.grid(pos)).astype(np.float32)
+    for step in range(1, 3):
+        g.step(step)
+        o.sweep(pos, om, step - 1)
+        om = o.omega(o.grid(pos)).astype(np.float32)
+    assert same_bits(g.positions(), pos)
+    assert same_bits(g.omega(), om)
+    g.close()
+
+
+def test_set_rng_replays_a_sweep():
+    banner("set_rng replay")
+    spec = W.tiny()
+    g = scmf.System(spec, make_opts(seed=4))
+    g.grid_density()
+    g.update_omega()
+    start = g.positions()
+    g.mc_sweep(1)
+    once = g.positions()
+    g.set_positions(start)
+    g.set_rng(4, 0)
+    g.mc_sweep(1)
+    assert same_bits(g.positions(), once)
+    g.set_positions(start)
+    g.set_rng(5, 0)
+    g.mc_sweep(1)
+    assert not same_bits(g.positions(), once)
+    g.close()
+
+
+def test_large_melt_invariants():
+    """268M beads, 256^3 grid, uint16: size-independent checks of init and gridding."""
+    banner("large diblock melt invariants")
+    spec = W.large_diblock_melt()
+    g = scmf.System(spec, make_opts(seed=0, density_bits=16))
+    g.grid_density()
+    c = g.density().reshape(2, -1)
+    assert c.sum() == spec.n_beads
+    assert c[0].sum() == 19 * 4_194_304 and c[1].sum() == 45 * 4_194_304
+    assert g.stats().overflow == 0
+    g.grid_density()                              # idempotent
+    np.testing.assert_array_equal(g.density().reshape(2, -1), c)
+    g.update_omega()
+    om = g.omega()
+    assert np.isfinite(om).all()
+    st = g.stats()
+    assert abs(st.re2 - 1.0) < 0.01               # Gaussian walks: <Re^2> = 1
+    # a spot check of a sweep on a slice: the first 2048 chains against the oracle
+    pos = g.positions()
+    sl = 2048 * 64
+    sub = SystemSpec(n_types=2, xn=spec.xn, L=spec.L, n=spec.n, dtNxi=spec.dtNxi, nref=64,
+                     polymers=[W.diblock(64, 0.3, 2048)])
+    o = orc.Oracle(sub, g.opts)
+    ref = np.ascontiguousarray(pos[:sl])
+    o.sweep(ref, om, 0)
+    g.mc_sweep(1)
+    assert same_bits(g.positions()[:sl], ref)
+    g.close()
+
+
+def test_copy_errors():
+    banner("copy errors")
+    g = scmf.System(W.tiny(), make_opts(seed=1))
+    with pytest.raises(scmf.ScmfError):
+        g.set_positions(np.zeros((3, 3), np.float32))
+    with pytest.raises(scmf.ScmfError):
+        g.set_omega(np.zeros(5, np.float32))
+    g.close()
+    g16 = scmf.System(W.tiny(), make_opts(seed=1, density_bits=16))
+    bad = np.zeros(2 * 512, np.uint32)
+    bad[3] = 70000
+    with pytest.raises(scmf.ScmfError):
+        g16.set_density(bad)
+    g16.close()
+
+
+def test_files_and_driver(golden, tmp_path):
+    banner("files + driver")
+    spec = W.reference_shipped()
+    g = scmf.System(spec, make_opts(seed=3))
+    g.grid_density()
+    g.update_omega()
+    g.write_density(10, str(tmp_path))
+    g.write_coord(str(tmp_path / "coord.dat"))
+    lines = (tmp_path / "density_10.dat").read_text().splitlines()
+    assert lines[0] == "Nx=24, Ny=24, Nz=24" and lines[1] == "dx=0.166667, dy=0.166667, dz=0.166667"
+    assert len(lines) == 2 + 24 ** 3
+    xyz = np.loadtxt(tmp_path / "coord.dat", dtype=np.float64)
+    np.testing.assert_allclose(xyz, g.positions(), rtol=1e-5, atol=1e-5)
+    g.close()
+    # the C++ driver on the reference's own input files
+    inp = os.path.join(golden, "reference_inputs")
+    for f in ("configuration.dat", "polymer.dat"):
+        (tmp_path / f).write_text(open(os.path.join(inp, f)).read())
+    exe = os.path.join(ROOT, "new_arch_amd", "lib", "scmf_run")
+    r = subprocess.run([exe, "-Steps_N=20", "-Check_T=10", "-seed=5"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "step 10 end to end dis" in r.stdout and "time =" in r.stdout
+    assert (tmp_path / "density_20.dat").exists()
