@@ -52,8 +52,8 @@ def cpu_baseline(spec, opts, target_s=15.0):
     o.sweep(pos, om, 0)
     dt = time.perf_counter() - t
     per_chain_sweep = dt / chains
-    sweeps = 2
-    chains = int(max(4096, min(spec.n_chains, target_s / (sweeps * per_chain_sweep))))
+    chains = int(max(4096, min(spec.n_chains, target_s / (2 * per_chain_sweep))))
+    sweeps = int(max(2, min(20, target_s / (chains * per_chain_sweep))))
     o = orc.Oracle(sub(chains), opts)
     pos = o.init_coords()
     t = time.perf_counter()
@@ -95,6 +95,8 @@ def main():
     g.update_omega()
     g.sync()
     log(f"created in {time.perf_counter() - t:.1f} s")
+    lanes, tuned_ms = g.autotune(2)
+    log(f"autotune: {lanes} lanes per chain, {tuned_ms:.3f} ms per sweep")
     for i in range(args.warmup):
         g.step(i + 1)
     g.sync()
@@ -145,7 +147,8 @@ def main():
         "config": {"workload": args.workload, "beads": B, "chains": spec.n_chains,
                    "N": [p.length for p in spec.polymers], "grid": list(spec.n),
                    "box_Re": list(spec.L), "xn": spec.xn, "move": move,
-                   "density_bits": bits, "omega": "every sweep", "parallelism": "1 GPU"},
+                   "density_bits": bits, "omega": "every sweep", "parallelism": "1 GPU",
+                   "lanes_per_chain": lanes},
         "roofline": dict(roof(sweep_bytes, tm.sweep_ms), kernel="k_sweep"),
         "passes": {"grid": dict(roof(grid_bytes, tm.grid_ms), kernel="k_zero+k_grid"),
                    "omega": dict(roof(omega_bytes, tm.omega_ms), kernel="k_omega")},

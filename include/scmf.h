@@ -72,6 +72,10 @@ typedef struct scmf_opts {
   int32_t init_coords;      /* 1: build the initial Gaussian walks on the device
                                (initialize_coord, device_function.cu:140-207);
                                0: the caller supplies positions (scmf_copy_in_pos) */
+  int32_t lanes_per_chain;  /* sweep kernel shape: 1, 2, 4 or 8 lanes share a chain
+                               staged in LDS; -1 = one lane per chain in global
+                               memory; 0 = pick automatically.  Results do not
+                               depend on it. */
 } scmf_opts;
 
 typedef struct scmf_stats_t {
@@ -155,6 +159,15 @@ int scmf_enable_timing(scmf_sys* sys, int32_t on);
 int scmf_timing(scmf_sys* sys, scmf_timing_t* out);
 int scmf_reset_timing(scmf_sys* sys);
 int scmf_sync(scmf_sys* sys);
+
+/* Autotuner for the sweep kernel shape (lanes per chain, LDS tile): times
+ * every variant that fits for a few sweeps on the system's own state, keeps
+ * the fastest, and restores positions, counters and the stream position, so
+ * tuning leaves the trajectory unchanged.  No counterpart in the reference,
+ * whose launch shape is the largest divisor of the chain count up to 1024
+ * (init_config.cu:8-64). */
+int scmf_autotune(scmf_sys* sys, int32_t sweeps_per_variant, int32_t* lanes_per_chain,
+                  float* ms_per_sweep);
 
 /* Sizes of the system (for host buffers). */
 int scmf_sizes(scmf_sys* sys, int64_t* n_beads, int64_t* n_chains, int64_t* n_cells,

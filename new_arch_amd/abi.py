@@ -50,6 +50,7 @@ class Opts(C.Structure):
         ("omega_every_step", C.c_int32),
         ("density_bits", C.c_int32),
         ("init_coords", C.c_int32),
+        ("lanes_per_chain", C.c_int32),
     ]
 
 
@@ -175,7 +176,7 @@ class SystemSpec:
 
 
 def make_opts(seed=1, system_id=0, move="smart", trial_delta=0.1, omega_every_step=False,
-              density_bits=32, init_coords=True) -> Opts:
+              density_bits=32, init_coords=True, lanes_per_chain=0) -> Opts:
     o = Opts()
     o.seed = int(seed)
     o.system_id = int(system_id)
@@ -184,4 +185,5 @@ def make_opts(seed=1, system_id=0, move="smart", trial_delta=0.1, omega_every_st
     o.omega_every_step = 1 if omega_every_step else 0
     o.density_bits = int(density_bits)
     o.init_coords = 1 if init_coords else 0
+    o.lanes_per_chain = int(lanes_per_chain)
     return o

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "scmf_bin.h"
 #include "scmf_prims.h"
 
 #pragma clang fp contract(off)
@@ -24,6 +25,7 @@ struct KParams {
   int nx, ny, nz;
   uint32_t n_cells;
   double Lx, Ly, Lz, dx, dy, dz;
+  Axis ax[3];                    // binning constants per axis
   double harm;                   // harmonic_normb
   double inv_nref;               // inverse_refbeads
   double fst[MAX_T];             // field_scaling_type
@@ -32,6 +34,8 @@ struct KParams {
   int64_t chain_base[MAX_P + 1]; // polymer_basis_gpu
   int64_t bead_base[MAX_P + 1];  // monomer_poly_basis_gpu
   int len[MAX_P];
+  int linear[MAX_P];             // 1: bonds are exactly i <-> i+1 (neighbours computed, not looked up)
+  double inv_len[MAX_P];         // 1/len, for the bead -> (chain, monomer) split
   int mono_off[MAX_P + 1];
   const int* monotype;           // [sum N_p]
   const int* nbr_start;          // [sum N_p + 1], CSR over every bead of every type
@@ -43,33 +47,11 @@ struct KParams {
   float trial_delta;
 };
 
-// coord_to_index_unified_high, device_function.cu:313-360 (one axis).
-// Coordinates the reference cannot bin (non-finite, or beyond 2^31 box
-// lengths, where its int conversions overflow) land in cell 0.
-__device__ __forceinline__ int fold_axis(double r, double L, double d, int n) {
-  int i = 0;
-  if (r >= 0) {
-    const double q = r / L;
-    if (!(q < 2147483647.0)) return 0;
-    const int k = (int)q;
-    i = (int)((r - L * k) / d);
-    i = i % n;
-  } else if (r < 0) {
-    const double q = -r / L;
-    if (!(q < 2147483646.0)) return 0;
-    const int k = (int)q + 1;
-    i = (int)((k * L + r) / d);
-    i = i % n;
-  }
-  if ((unsigned)i >= (unsigned)n) i = 0;
-  return i;
-}
-
 __device__ __forceinline__ uint32_t bin_cell(const KParams& kp, double rx, double ry, double rz,
                                              uint32_t type) {
-  const int xi = fold_axis(rx, kp.Lx, kp.dx, kp.nx);
-  const int yi = fold_axis(ry, kp.Ly, kp.dy, kp.ny);
-  const int zi = fold_axis(rz, kp.Lz, kp.dz, kp.nz);
+  const int xi = fold_fast(rx, kp.ax[0]);
+  const int yi = fold_fast(ry, kp.ax[1]);
+  const int zi = fold_fast(rz, kp.ax[2]);
   return type * kp.n_cells + (uint32_t)(xi + yi * kp.nx + zi * kp.nx * kp.ny);
 }
 

@@ -7,7 +7,10 @@
 //   omega          fp32 [type][z][y][x]
 // Kernels:
 //   k_init_coords  one lane per chain: start point + Gaussian walk
-//   k_sweep        one lane per chain: N sequential smart-MC / trial moves
+//   k_sweep_lds    one lane per chain, the wavefront's 64 chains staged in LDS:
+//                  N sequential smart-MC / trial moves per chain
+//   k_sweep_global the same with positions in global memory (chains too long
+//                  for an LDS tile)
 //   k_zero         clear the counts (a kernel, not a memset, so it can sit in a graph)
 //   k_grid         one lane per bead: NGP cell, atomic increment
 //   k_omega        one lane per cell: all T omega values in fp64, stored fp32
@@ -68,119 +71,324 @@ __global__ void __launch_bounds__(BLOCK) k_init_coords(KParams kp, float* __rest
   }
 }
 
-// add_bond_forces_arr, device_function.cu:589-634.
-__device__ __forceinline__ void bond_forces(const KParams& kp, int b, int64_t poly_index,
-                                            const float* x, const float* y, const float* z,
-                                            float px, float py, float pz, float* fx, float* fy,
-                                            float* fz) {
+// Positions of one chain, seen by one lane.
+struct GlobalChain {  // planar global arrays, chain starting at `base`
+  float* x;
+  float* y;
+  float* z;
+  __device__ float X(int b) const { return x[b]; }
+  __device__ float Y(int b) const { return y[b]; }
+  __device__ float Z(int b) const { return z[b]; }
+  __device__ void set(int b, float a, float c, float d) const {
+    x[b] = a;
+    y[b] = c;
+    z[b] = d;
+  }
+};
+template <int STRIDE>
+struct LdsChain {  // LDS columns: bead b of this lane's chain at [b * STRIDE]
+  float* x;
+  float* y;
+  float* z;
+  __device__ float X(int b) const { return x[b * STRIDE]; }
+  __device__ float Y(int b) const { return y[b * STRIDE]; }
+  __device__ float Z(int b) const { return z[b * STRIDE]; }
+  __device__ void set(int b, float a, float c, float d) const {
+    x[b * STRIDE] = a;
+    y[b * STRIDE] = c;
+    z[b * STRIDE] = d;
+  }
+};
+
+// Calls f(j) for every neighbour j of bead b, ascending (the order of
+// init_config.cu:330-336): computed for linear chains, looked up otherwise.
+template <class F>
+__device__ __forceinline__ void for_neighbours(const KParams& kp, int p, int b, int len, F&& f) {
+  if (kp.linear[p]) {
+    if (b > 0) f(b - 1);
+    if (b < len - 1) f(b + 1);
+  } else {
+    const int k0 = kp.nbr_start[kp.mono_off[p] + b], k1 = kp.nbr_start[kp.mono_off[p] + b + 1];
+    for (int k = k0; k < k1; ++k) f(kp.nbr[k]);
+  }
+}
+
+// add_bond_forces_arr, device_function.cu:589-634: neighbour difference in
+// fp32, scaled and accumulated in fp64, added to the fp32 force.
+template <class V>
+__device__ __forceinline__ void bond_force(const KParams& kp, const V& c, int p, int b, int len,
+                                           float px, float py, float pz, float* fx, float* fy,
+                                           float* fz) {
   const double harm = kp.harm;
   double v1x = 0.0, v1y = 0.0, v1z = 0.0;
-  const int k0 = kp.nbr_start[b], k1 = kp.nbr_start[b + 1];
-  for (int k = k0; k < k1; ++k) {
-    const int64_t ni = poly_index + kp.nbr[k];
-    v1x += (double)(x[ni] - px) * 2.0 * harm;
-    v1y += (double)(y[ni] - py) * 2.0 * harm;
-    v1z += (double)(z[ni] - pz) * 2.0 * harm;
-  }
+  for_neighbours(kp, p, b, len, [&](int j) {
+    v1x += (double)(c.X(j) - px) * 2.0 * harm;
+    v1y += (double)(c.Y(j) - py) * 2.0 * harm;
+    v1z += (double)(c.Z(j) - pz) * 2.0 * harm;
+  });
   *fx = (float)((double)*fx + v1x);
   *fy = (float)((double)*fy + v1y);
   *fz = (float)((double)*fz + v1z);
 }
 
-// mc_polymer_move_arr, device_function.cu:718-799, one lane per chain, with
+constexpr int MAX_DEG = 8;  // bonds per bead handled by the sweep kernels
+
+// One move of mc_polymer_move_arr (device_function.cu:747-790) with
 // trial_move_smc_arr (:637-717), calc_delta_energy_arr (:562-588),
 // calc_delta_bonded_energy_arr (:509-561) and som_accept (:444-454) inlined.
-__global__ void __launch_bounds__(BLOCK) k_sweep(KParams kp, float* __restrict__ x,
-                                                 float* __restrict__ y, float* __restrict__ z,
-                                                 const float* __restrict__ omega, uint32_t sweep,
-                                                 unsigned long long* __restrict__ accepts) {
+// Returns 1 if accepted.
+template <class V>
+__device__ __forceinline__ uint32_t one_move(const KParams& kp, const V& c, int p, int length,
+                                             uint32_t chain, uint32_t sweep, uint32_t i,
+                                             const float* __restrict__ omega) {
+  const u32x4 pick = stream_block(kp.seed, kp.system_id, chain, sweep, i, SLOT_PICK);
+  int ibead = (int)(u01_53(pick.v[0], pick.v[1]) * length);
+  if (ibead == length) ibead = length - 1;
+  const uint32_t iwtype = (uint32_t)kp.monotype[kp.mono_off[p] + ibead];
+  const u32x4 disp = stream_block(kp.seed, kp.system_id, chain, sweep, i, SLOT_DISP);
+  const float ox = c.X(ibead), oy = c.Y(ibead), oz = c.Z(ibead);
+  float dx, dy, dz;
+  double smc_deltaE = 0.0;
+  if (kp.move_type == SCMF_MOVE_SMART) {
+    float px = ox, py = oy, pz = oz;
+    const float A = kp.A[iwtype];
+    const float R = kp.R[iwtype];
+    float fx = 0.0f, fy = 0.0f, fz = 0.0f;
+    bond_force(kp, c, p, ibead, length, px, py, pz, &fx, &fy, &fz);
+    double n0, n1, n2;
+    normal3(disp, &n0, &n1, &n2);
+    const float rx = (float)n0, ry = (float)n1, rz = (float)n2;
+    dx = A * fx + rx * R;
+    dy = A * fy + ry * R;
+    dz = A * fz + rz * R;
+    px += dx;
+    py += dy;
+    pz += dz;
+    float nfx = 0.0f, nfy = 0.0f, nfz = 0.0f;
+    bond_force(kp, c, p, ibead, length, px, py, pz, &nfx, &nfy, &nfz);
+    smc_deltaE = 0.0;
+    smc_deltaE += 0.5 * ((nfx + fx) * dx + (nfy + fy) * dy + (nfz + fz) * dz);
+    smc_deltaE +=
+        0.25 * A * ((nfx * nfx) + (nfy * nfy) + (nfz * nfz) - (fx * fx) - (fy * fy) - (fz * fz));
+  } else {
+    const float delta = kp.trial_delta;
+    dx = delta * (2.0f * u01_24f_closed0(disp.v[0]) - 1.0f);
+    dy = delta * (2.0f * u01_24f_closed0(disp.v[1]) - 1.0f);
+    dz = delta * (2.0f * u01_24f_closed0(disp.v[2]) - 1.0f);
+  }
+  const float newx = ox + dx, newy = oy + dy, newz = oz + dz;
+  const double xold = ox, yold = oy, zold = oz;
+  const uint32_t cell_old = bin_cell(kp, xold, yold, zold, iwtype);
+  const uint32_t cell_new =
+      bin_cell(kp, xold + (double)dx, yold + (double)dy, zold + (double)dz, iwtype);
+  const double energy_old = omega[cell_old];
+  const double energy_new = omega[cell_new];
+  double bonded = 0;
+  {
+    const double ddx = dx, ddy = dy, ddz = dz;
+    for_neighbours(kp, p, ibead, length, [&](int j) {
+      const double old_rx = (double)(ox - c.X(j));
+      const double new_rx = old_rx + ddx;
+      const double old_ry = (double)(oy - c.Y(j));
+      const double new_ry = old_ry + ddy;
+      const double old_rz = (double)(oz - c.Z(j));
+      const double new_rz = old_rz + ddz;
+      const double old_r2 = old_rx * old_rx + old_ry * old_ry + old_rz * old_rz;
+      const double new_r2 = new_rx * new_rx + new_ry * new_ry + new_rz * new_rz;
+      bonded += kp.harm * (new_r2 - old_r2);
+    });
+  }
+  double energy = energy_new - energy_old;
+  energy += bonded;
+  double delta_energy = energy;
+  delta_energy += smc_deltaE;
+  const double p_acc = exp_cw(-1.0 * delta_energy);
+  if ((p_acc > 1) || (p_acc > (double)u01_24f(pick.v[2]))) {
+    c.set(ibead, newx, newy, newz);
+    return 1u;
+  }
+  return 0u;
+}
+
+__device__ __forceinline__ void wave_count(uint32_t acc, unsigned long long* accepts) {
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(accepts, (unsigned long long)acc);
+}
+
+// mc_polymer_move_arr, one lane per chain, positions read and written in
+// global memory (used when a chain tile does not fit in LDS).
+__global__ void __launch_bounds__(BLOCK) k_sweep_global(KParams kp, float* __restrict__ x,
+                                                        float* __restrict__ y,
+                                                        float* __restrict__ z,
+                                                        const float* __restrict__ omega,
+                                                        uint32_t sweep,
+                                                        unsigned long long* __restrict__ accepts) {
   const int64_t chain = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   uint32_t acc = 0;
   if (chain < kp.n_chains) {
     const int p = arch_of_chain(kp, chain);
     const int length = kp.len[p];
-    const int64_t poly_index = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
-    const int mo = kp.mono_off[p];
-    for (int i = 0; i < length; i++) {
-      const u32x4 pick =
-          stream_block(kp.seed, kp.system_id, (uint32_t)chain, sweep, (uint32_t)i, SLOT_PICK);
-      int ibead = (int)(u01_53(pick.v[0], pick.v[1]) * length);
-      if (ibead == length) ibead = length - 1;
-      const uint32_t iwtype = (uint32_t)kp.monotype[mo + ibead];
-      const int64_t index = poly_index + ibead;
-      const int b = mo + ibead;
-      const u32x4 disp =
-          stream_block(kp.seed, kp.system_id, (uint32_t)chain, sweep, (uint32_t)i, SLOT_DISP);
-      const float ox = x[index], oy = y[index], oz = z[index];
-      float dx, dy, dz;
-      double smc_deltaE = 0.0;
-      if (kp.move_type == SCMF_MOVE_SMART) {
-        float px = ox, py = oy, pz = oz;
-        const float A = kp.A[iwtype];
-        const float R = kp.R[iwtype];
-        float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-        bond_forces(kp, b, poly_index, x, y, z, px, py, pz, &fx, &fy, &fz);
-        double n0, n1, n2;
-        normal3(disp, &n0, &n1, &n2);
-        const float rx = (float)n0, ry = (float)n1, rz = (float)n2;
-        dx = A * fx + rx * R;
-        dy = A * fy + ry * R;
-        dz = A * fz + rz * R;
-        px += dx;
-        py += dy;
-        pz += dz;
-        float nfx = 0.0f, nfy = 0.0f, nfz = 0.0f;
-        bond_forces(kp, b, poly_index, x, y, z, px, py, pz, &nfx, &nfy, &nfz);
-        smc_deltaE = 0.0;
-        smc_deltaE += 0.5 * ((nfx + fx) * dx + (nfy + fy) * dy + (nfz + fz) * dz);
-        smc_deltaE += 0.25 * A *
-                      ((nfx * nfx) + (nfy * nfy) + (nfz * nfz) - (fx * fx) - (fy * fy) - (fz * fz));
-      } else {
-        const float delta = kp.trial_delta;
-        dx = delta * (2.0f * u01_24f_closed0(disp.v[0]) - 1.0f);
-        dy = delta * (2.0f * u01_24f_closed0(disp.v[1]) - 1.0f);
-        dz = delta * (2.0f * u01_24f_closed0(disp.v[2]) - 1.0f);
-      }
-      const float newx = ox + dx, newy = oy + dy, newz = oz + dz;
-      const double xold = ox, yold = oy, zold = oz;
-      const uint32_t cell_old = bin_cell(kp, xold, yold, zold, iwtype);
-      const double energy_old = omega[cell_old];
-      const uint32_t cell_new =
-          bin_cell(kp, xold + (double)dx, yold + (double)dy, zold + (double)dz, iwtype);
-      const double energy_new = omega[cell_new];
-      double bonded = 0;
-      {
-        const double ddx = dx, ddy = dy, ddz = dz;
-        const int k0 = kp.nbr_start[b], k1 = kp.nbr_start[b + 1];
-        for (int k = k0; k < k1; ++k) {
-          const int64_t nb = poly_index + kp.nbr[k];
-          const double old_rx = (double)(ox - x[nb]);
-          const double new_rx = old_rx + ddx;
-          const double old_ry = (double)(oy - y[nb]);
-          const double new_ry = old_ry + ddy;
-          const double old_rz = (double)(oz - z[nb]);
-          const double new_rz = old_rz + ddz;
-          const double old_r2 = old_rx * old_rx + old_ry * old_ry + old_rz * old_rz;
-          const double new_r2 = new_rx * new_rx + new_ry * new_ry + new_rz * new_rz;
-          bonded += kp.harm * (new_r2 - old_r2);
-        }
-      }
-      double energy = energy_new - energy_old;
-      energy += bonded;
-      double delta_energy = energy;
-      delta_energy += smc_deltaE;
-      const double p_acc = exp_cw(-1.0 * delta_energy);
-      if ((p_acc > 1) || (p_acc > (double)u01_24f(pick.v[2]))) {
-        x[index] = newx;
-        y[index] = newy;
-        z[index] = newz;
-        ++acc;
-      }
+    const int64_t off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
+    const GlobalChain c{x + off, y + off, z + off};
+    for (int i = 0; i < length; i++)
+      acc += one_move(kp, c, p, length, (uint32_t)chain, sweep, (uint32_t)i, omega);
+  }
+  wave_count(acc, accepts);
+}
+
+// mc_polymer_move_arr with the wavefront's 64 chains staged in LDS: one lane
+// per chain, bead b of lane l at LDS word b*64 + l (conflict-free for any
+// per-lane bead index).  Each lane touches only its own column, so no barrier
+// is needed; global memory sees one read and one write of every coordinate.
+__global__ void __launch_bounds__(64) k_sweep_lds(KParams kp, float* __restrict__ x,
+                                                  float* __restrict__ y, float* __restrict__ z,
+                                                  const float* __restrict__ omega,
+                                                  uint32_t sweep, int nmax,
+                                                  unsigned long long* __restrict__ accepts) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const int64_t chain = (int64_t)blockIdx.x * 64 + lane;
+  uint32_t acc = 0;
+  if (chain < kp.n_chains) {
+    const int p = arch_of_chain(kp, chain);
+    const int length = kp.len[p];
+    const int64_t off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
+    float* sx = lds + lane;
+    float* sy = lds + nmax * 64 + lane;
+    float* sz = lds + 2 * nmax * 64 + lane;
+    for (int b = 0; b < length; ++b) {
+      sx[b * 64] = x[off + b];
+      sy[b * 64] = y[off + b];
+      sz[b * 64] = z[off + b];
+    }
+    const LdsChain<64> c{sx, sy, sz};
+    for (int i = 0; i < length; i++)
+      acc += one_move(kp, c, p, length, (uint32_t)chain, sweep, (uint32_t)i, omega);
+    for (int b = 0; b < length; ++b) {
+      x[off + b] = sx[b * 64];
+      y[off + b] = sy[b * 64];
+      z[off + b] = sz[b * 64];
     }
   }
-  // Wavefront sum of accepted moves, one atomic per wavefront.
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(accepts, (unsigned long long)acc);
+  wave_count(acc, accepts);
+}
+
+// mc_polymer_move_arr with L lanes per chain (G = 64/L chains per wavefront).
+//
+// The N moves of a chain in one sweep are not all dependent: move i reads the
+// closed neighbourhood {b_i} + bonded(b_i) and writes b_i, so it depends on an
+// earlier move j only if b_j lies in that neighbourhood.  The picks b_i come
+// from the counter streams before any move runs, so each chain's dependency
+// levels are computed first (level_i = 1 + max level of earlier moves that
+// wrote a bead in i's neighbourhood); moves of one level are independent and
+// run on the chain's L lanes side by side, levels in order.  Every move sees
+// exactly the state it sees in the sequential order, so results are
+// bit-identical to one lane per chain; the per-wavefront LDS tile shrinks by L.
+//
+// LDS per chain (index type I: uint8 for N <= 255):
+//   positions 3 N floats (bead b at [b*G + slot]), lev N, W N+1, cnt N+1, sched N*L
+template <int L, typename I>
+__global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restrict__ x,
+                                                    float* __restrict__ y,
+                                                    float* __restrict__ z,
+                                                    const float* __restrict__ omega,
+                                                    uint32_t sweep, int nmax,
+                                                    unsigned long long* __restrict__ accepts) {
+  constexpr int G = 64 / L;
+  constexpr I EMPTY = (I)~(I)0;
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x;
+  const int slot = lane / L, r = lane % L;
+  float* sx = lds;
+  float* sy = lds + nmax * G;
+  float* sz = lds + 2 * nmax * G;
+  I* book = (I*)(lds + 3 * nmax * G);
+  const int per = 3 * nmax + 2 + nmax * L;  // lev, W, cnt, sched
+  I* lev = book + slot * per;
+  I* W = lev + nmax;
+  I* cnt = W + nmax + 1;
+  I* sched = cnt + nmax + 1;
+  const int64_t chain = (int64_t)blockIdx.x * G + slot;
+  const bool active = chain < kp.n_chains;
+  int p = 0, length = 0;
+  int64_t off = 0;
+  if (active) {
+    p = arch_of_chain(kp, chain);
+    length = kp.len[p];
+    off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
+  }
+  // load the tile and draw the picks
+  for (int b = r; b < length; b += L) {
+    sx[b * G + slot] = x[off + b];
+    sy[b * G + slot] = y[off + b];
+    sz[b * G + slot] = z[off + b];
+  }
+  for (int i = r; i < length; i += L) {
+    const u32x4 pick = stream_block(kp.seed, kp.system_id, (uint32_t)chain, sweep, (uint32_t)i,
+                                    SLOT_PICK);
+    int ib = (int)(u01_53(pick.v[0], pick.v[1]) * length);
+    if (ib == length) ib = length - 1;
+    lev[i] = (I)ib;
+    W[i] = 0;
+    cnt[i] = 0;
+  }
+  for (int k = r; k < length * L; k += L) sched[k] = EMPTY;
+  if (r == 0) {
+    cnt[length] = 0;
+    W[length] = 0;
+  }
+  __syncthreads();
+  // dependency levels (one lane per chain), then step offsets and the schedule
+  int steps = 0;
+  if (r == 0 && active) {
+    int maxl = 0;
+    for (int i = 0; i < length; ++i) {
+      const int b = lev[i];
+      int l = W[b];
+      for_neighbours(kp, p, b, length, [&](int j) { l = max(l, (int)W[j]); });
+      l += 1;
+      W[b] = (I)l;
+      lev[i] = (I)l;
+      cnt[l] = (I)(cnt[l] + 1);
+      maxl = max(maxl, l);
+    }
+    for (int l = 1; l <= maxl; ++l) {  // cnt[l] -> first step of level l
+      const int c = cnt[l];
+      cnt[l] = (I)steps;
+      steps += (c + L - 1) / L;
+      W[l] = 0;                        // W reused as the per-level fill counter
+    }
+    for (int i = 0; i < length; ++i) {
+      const int l = lev[i];
+      const int k = W[l];
+      W[l] = (I)(k + 1);
+      sched[((int)cnt[l] + k / L) * L + k % L] = (I)i;
+    }
+  }
+  // all lanes of the wavefront run max(steps) steps
+  int wsteps = steps;
+  for (int o = 32; o > 0; o >>= 1) wsteps = max(wsteps, __shfl_xor(wsteps, o, 64));
+  __syncthreads();
+  uint32_t acc = 0;
+  const LdsChain<G> c{sx + slot, sy + slot, sz + slot};
+  for (int st = 0; st < wsteps; ++st) {
+    const int i = active && st * L < length * L ? (int)sched[st * L + r] : (int)EMPTY;
+    if (i != (int)EMPTY)
+      acc += one_move(kp, c, p, length, (uint32_t)chain, sweep, (uint32_t)i, omega);
+    __syncthreads();
+  }
+  for (int b = r; b < length; b += L) {
+    x[off + b] = sx[b * G + slot];
+    y[off + b] = sy[b * G + slot];
+    z[off + b] = sz[b * G + slot];
+  }
+  wave_count(acc, accepts);
+}
+
+template <int L, typename I>
+size_t sched_lds_bytes(int nmax) {
+  constexpr int G = 64 / L;
+  return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (3 * nmax + 2 + nmax * L) * sizeof(I);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_zero(uint32_t* __restrict__ p, int64_t n) {
@@ -198,7 +406,13 @@ __global__ void __launch_bounds__(BLOCK) k_grid(KParams kp, const float* __restr
   const int64_t stride = (int64_t)gridDim.x * BLOCK;
   for (int64_t g = (int64_t)blockIdx.x * BLOCK + threadIdx.x; g < kp.n_beads; g += stride) {
     const int p = arch_of_bead(kp, g);
-    const int m = (int)((g - kp.bead_base[p]) % kp.len[p]);
+    // monomer index rel mod len: quotient from one fp64 product, which is
+    // at most one below the true quotient; the correction makes it exact
+    const uint32_t rel = (uint32_t)(g - kp.bead_base[p]);
+    const uint32_t q = (uint32_t)((double)rel * kp.inv_len[p]);
+    int m = (int)(rel - q * (uint32_t)kp.len[p]);
+    if (m >= kp.len[p]) m -= kp.len[p];
+    if (m < 0) m += kp.len[p];
     const uint32_t t = (uint32_t)kp.monotype[kp.mono_off[p] + m];
     const uint32_t cell = bin_cell(kp, (double)x[g], (double)y[g], (double)z[g], t);
     if (BITS == 32) {
@@ -325,6 +539,14 @@ int fail(int code, const std::string& msg) {
 
 enum Kind { K_SWEEP = 0, K_GRID = 1, K_OMEGA = 2, K_KINDS = 3 };
 
+// Sweep kernel variants: L lanes per chain (0 = positions in global memory).
+struct SweepVariant {
+  int L;
+  bool wide;  // uint16 bookkeeping (chains longer than 255 beads)
+  const void* fn;
+  size_t (*bytes)(int nmax);
+};
+
 struct Timed {
   int kind;
   hipEvent_t a, b;
@@ -354,6 +576,9 @@ struct scmf_sys {
   KParams kp{};
   int64_t n_beads = 0, n_chains = 0, n_cells = 0;
   int T = 0;
+  int nmax = 0;         // longest chain
+  size_t lds_bytes = 0; // LDS of the sweep kernel
+  const SweepVariant* variant = nullptr;
   float* x = nullptr;
   float* y = nullptr;
   float* z = nullptr;
@@ -427,6 +652,9 @@ int derive(scmf_sys* s) {
   kp.dx = kp.Lx / kp.nx;
   kp.dy = kp.Ly / kp.ny;
   kp.dz = kp.Lz / kp.nz;
+  kp.ax[0] = make_axis(kp.Lx, kp.nx);
+  kp.ax[1] = make_axis(kp.Ly, kp.ny);
+  kp.ax[2] = make_axis(kp.Lz, kp.nz);
   s->n_cells = (int64_t)kp.nx * kp.ny * kp.nz;
   s->T = d.T;
   if ((int64_t)d.T * s->n_cells >= (int64_t)0xFFFFFFFF)
@@ -445,6 +673,8 @@ int derive(scmf_sys* s) {
     kp.bead_base[p] = bb;
     kp.mono_off[p] = mo;
     kp.len[p] = d.poly_length[p];
+    kp.inv_len[p] = 1.0 / (double)d.poly_length[p];
+    s->nmax = std::max(s->nmax, (int)d.poly_length[p]);
     cb += d.n_polymers[p];
     bb += d.n_polymers[p] * (int64_t)d.poly_length[p];
     mo += d.poly_length[p];
@@ -476,11 +706,21 @@ int upload_arch(scmf_sys* s) {
   size_t coff = 0;
   for (size_t p = 0; p < d.poly_length.size(); ++p) {
     const int N = d.poly_length[p];
+    bool linear = true;
     for (int i = 0; i < N; ++i) {
       nbr_start.push_back((int)nbr.size());
-      for (int j = 0; j < N; ++j)
-        if (d.connection[coff + (size_t)i * N + j]) nbr.push_back(j);
+      int deg = 0;
+      for (int j = 0; j < N; ++j) {
+        const bool bonded = d.connection[coff + (size_t)i * N + j] != 0;
+        if (bonded) {
+          nbr.push_back(j);
+          ++deg;
+        }
+        if (bonded != (j == i - 1 || j == i + 1)) linear = false;
+      }
+      if (deg > MAX_DEG) return fail(SCMF_E_ARG, "a bead has more than 8 bonds");
     }
+    s->kp.linear[p] = linear ? 1 : 0;
     coff += (size_t)N * N;
   }
   nbr_start.push_back((int)nbr.size());
@@ -510,6 +750,88 @@ int ensure_staging(scmf_sys* s, int64_t beads) {
 }
 
 constexpr int64_t STAGE_BEADS = (int64_t)1 << 24;
+
+size_t lds1_bytes(int nmax) { return (size_t)3 * nmax * 64 * sizeof(float); }
+size_t none_bytes(int) { return 0; }
+
+const SweepVariant kVariants[] = {
+    {0, false, (const void*)k_sweep_global, none_bytes},
+    {1, false, (const void*)k_sweep_lds, lds1_bytes},
+    {2, false, (const void*)k_sweep_sched<2, uint8_t>, sched_lds_bytes<2, uint8_t>},
+    {4, false, (const void*)k_sweep_sched<4, uint8_t>, sched_lds_bytes<4, uint8_t>},
+    {8, false, (const void*)k_sweep_sched<8, uint8_t>, sched_lds_bytes<8, uint8_t>},
+    {2, true, (const void*)k_sweep_sched<2, uint16_t>, sched_lds_bytes<2, uint16_t>},
+    {4, true, (const void*)k_sweep_sched<4, uint16_t>, sched_lds_bytes<4, uint16_t>},
+    {8, true, (const void*)k_sweep_sched<8, uint16_t>, sched_lds_bytes<8, uint16_t>},
+};
+
+// Pick the variant: the requested lanes_per_chain if it fits, else the
+// smallest L that keeps at least 8 wavefronts per CU resident by LDS, else
+// the one with the most resident wavefronts.
+int choose_sweep(scmf_sys* s) {
+  int max_block = 0, per_cu = 0;
+  (void)hipDeviceGetAttribute(&max_block, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
+  (void)hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
+                              s->device);
+  if (per_cu <= 0) per_cu = max_block;
+  const bool wide = s->nmax > 255;
+  const SweepVariant* best = &kVariants[0];
+  int best_waves = 0;
+  if (s->opts.lanes_per_chain == -1) {  // positions in global memory, one lane per chain
+    s->variant = &kVariants[0];
+    s->lds_bytes = 0;
+    return SCMF_OK;
+  }
+  for (const SweepVariant& v : kVariants) {
+    if (v.L == 0 || (v.L >= 2 && v.wide != wide)) continue;
+    const size_t need = v.bytes(s->nmax);
+    if (need > (size_t)max_block) continue;
+    const int waves = need ? (int)(per_cu / need) : 64;
+    if (s->opts.lanes_per_chain == v.L) {
+      best = &v;
+      break;
+    }
+    if (s->opts.lanes_per_chain == 0) {
+      const bool good = waves >= 8, best_good = best_waves >= 8;
+      if (best->L == 0 || (good && !best_good) || (!good && !best_good && waves > best_waves)) {
+        best = &v;
+        best_waves = waves;
+      }
+    }
+  }
+  if (s->opts.lanes_per_chain != 0 && best->L != s->opts.lanes_per_chain)
+    return fail(SCMF_E_ARG, "lanes_per_chain must be 1, 2, 4 or 8 and fit in LDS");
+  const size_t need = best->bytes(s->nmax);
+  if (need > 65536) {
+    hipError_t e = hipFuncSetAttribute(best->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      best = &kVariants[0];
+    }
+  }
+  s->variant = best;
+  s->lds_bytes = best->bytes(s->nmax);
+  return SCMF_OK;
+}
+
+int launch_sweep(scmf_sys* s) {
+  const SweepVariant* v = s->variant;
+  const uint32_t sweep = (uint32_t)s->sweeps;
+  if (v->L == 0) {
+    hipLaunchKernelGGL(k_sweep_global, dim3(grid_for(s->n_chains)), dim3(BLOCK), 0, s->stream,
+                       s->kp, s->x, s->y, s->z, s->omega, sweep, s->d_accepts);
+  } else {
+    const int G = 64 / v->L;
+    const dim3 grid((unsigned)((s->n_chains + G - 1) / G));
+    void* args[] = {(void*)&s->kp, (void*)&s->x, (void*)&s->y, (void*)&s->z,
+                    (void*)&s->omega, (void*)&sweep, (void*)&s->nmax, (void*)&s->d_accepts};
+    HIP_TRY(hipLaunchKernel(v->fn, grid, dim3(64), args, s->lds_bytes, s->stream));
+  }
+  HIP_TRY(hipGetLastError());
+  return SCMF_OK;
+}
+
+
 
 int free_all(scmf_sys* s) {
   for (void* p : {(void*)s->x, (void*)s->y, (void*)s->z, (void*)s->counts, (void*)s->omega,
@@ -567,6 +889,7 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(SCMF_E_HIP, "hipStreamCreate failed"));
   if ((rc = upload_arch(s))) return bail(rc);
+  if ((rc = choose_sweep(s))) return bail(rc);
   const int64_t B = s->n_beads;
   const int64_t TC = (int64_t)s->T * s->n_cells;
   s->count_words = o.density_bits == 32 ? TC : (TC + 1) / 2;
@@ -666,8 +989,8 @@ int scmf_mc_sweep(scmf_sys* s, int32_t n_sweeps) {
       int rc = s->ev_begin(K_SWEEP, &t);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_sweep, dim3(grid_for(s->n_chains)), dim3(BLOCK), 0, s->stream, s->kp,
-                       s->x, s->y, s->z, s->omega, (uint32_t)s->sweeps, s->d_accepts);
+    int rc = launch_sweep(s);
+    if (rc) return rc;
     HIP_TRY(hipGetLastError());
     if (s->timing) {
       int rc = s->ev_end(&t);
@@ -883,6 +1206,71 @@ int scmf_reset_timing(scmf_sys* s) {
     s->t_sum[k] = 0;
     s->t_n[k] = 0;
   }
+  return SCMF_OK;
+}
+
+int scmf_autotune(scmf_sys* s, int32_t sweeps_per_variant, int32_t* chosen, float* best_ms) {
+  if (!s) return fail(SCMF_E_ARG, "null handle");
+  if (sweeps_per_variant <= 0) sweeps_per_variant = 2;
+  const int64_t B = s->n_beads;
+  float* backup = nullptr;
+  if (hipMalloc(&backup, sizeof(float) * 3 * B) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(SCMF_E_NOMEM, "autotune needs a copy of the positions");
+  }
+  unsigned long long acc0 = 0;
+  HIP_TRY(hipMemcpyAsync(&acc0, s->d_accepts, sizeof acc0, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipMemcpyAsync(backup, s->x, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(backup + B, s->y, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(backup + 2 * B, s->z, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
+  const uint64_t sweeps0 = s->sweeps, moves0 = s->moves;
+  const SweepVariant* keep = s->variant;
+  const size_t keep_lds = s->lds_bytes;
+  const int32_t keep_req = s->opts.lanes_per_chain;
+  const SweepVariant* best = keep;
+  size_t best_lds = keep_lds;
+  float best_t = 1e30f;
+  hipEvent_t a, b;
+  HIP_TRY(hipEventCreate(&a));
+  HIP_TRY(hipEventCreate(&b));
+  int rc = SCMF_OK;
+  for (int32_t L : {-1, 1, 2, 4, 8}) {
+    s->opts.lanes_per_chain = L;
+    if (choose_sweep(s) != SCMF_OK) continue;
+    if ((rc = launch_sweep(s))) break;  // warm-up launch
+    HIP_TRY(hipEventRecord(a, s->stream));
+    for (int32_t k = 0; k < sweeps_per_variant && !rc; ++k) rc = launch_sweep(s);
+    HIP_TRY(hipEventRecord(b, s->stream));
+    HIP_TRY(hipEventSynchronize(b));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, a, b));
+    ms /= (float)sweeps_per_variant;
+    if (ms < best_t) {
+      best_t = ms;
+      best = s->variant;
+      best_lds = s->lds_bytes;
+    }
+    HIP_TRY(hipMemcpyAsync(s->x, backup, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->y, backup + B, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
+    HIP_TRY(hipMemcpyAsync(s->z, backup + 2 * B, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  HIP_TRY(hipMemcpyAsync(s->d_accepts, &acc0, sizeof acc0, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  (void)hipFree(backup);
+  s->sweeps = sweeps0;
+  s->moves = moves0;
+  s->opts.lanes_per_chain = keep_req;
+  if (rc) {
+    s->variant = keep;
+    s->lds_bytes = keep_lds;
+    return rc;
+  }
+  s->variant = best;
+  s->lds_bytes = best_lds;
+  if (chosen) *chosen = best->L == 0 ? -1 : best->L;
+  if (best_ms) *best_ms = best_t;
   return SCMF_OK;
 }
 

@@ -57,6 +57,7 @@ def lib():
     L.scmf_inspect_desc.argtypes = [C.POINTER(Desc), C.POINTER(Summary)]
     L.scmf_write_density_host.argtypes = [C.c_char_p, C.POINTER(C.c_int32),
                                           C.POINTER(C.c_double), C.c_int32, vp, vp, C.c_double]
+    L.scmf_autotune.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
     L.scmf_last_error.restype = C.c_char_p
     L.scmf_version.restype = C.c_char_p
     _lib = L
@@ -172,6 +173,13 @@ class System:
 
     def set_rng(self, seed, next_sweep):
         _check(lib().scmf_set_rng(self._h, int(seed), int(next_sweep)))
+
+    def autotune(self, sweeps_per_variant=2):
+        """Pick the fastest sweep kernel shape; returns (lanes_per_chain, ms/sweep)."""
+        lanes, ms = C.c_int32(), C.c_float()
+        _check(lib().scmf_autotune(self._h, int(sweeps_per_variant), C.byref(lanes),
+                                   C.byref(ms)))
+        return lanes.value, ms.value
 
     # -- diagnostics
     def stats(self) -> Stats:

@@ -113,12 +113,13 @@ def test_omega_reference_dumps_on_gpu(golden, step):
     g.close()
 
 
+@pytest.mark.parametrize("lanes", [0, -1, 1, 2, 4, 8])
 @pytest.mark.parametrize("move", ["smart", "trial"])
 @pytest.mark.parametrize("name", list(SPECS))
-def test_sweep_bit_exact(name, move):
-    banner(f"sweep {name} {move}")
+def test_sweep_bit_exact(name, move, lanes):
+    banner(f"sweep {name} {move} lanes={lanes}")
     spec = SPECS[name]()
-    g, o = pair(spec, seed=2024, move=move, trial_delta=0.1)
+    g, o = pair(spec, seed=2024, move=move, trial_delta=0.1, lanes_per_chain=lanes)
     pos = o.init_coords()
     g.grid_density()
     g.update_omega()
@@ -172,6 +173,24 @@ def test_small_diblock_melt_two_steps():
         om = o.omega(o.grid(pos)).astype(np.float32)
     assert same_bits(g.positions(), pos)
     assert same_bits(g.omega(), om)
+    g.close()
+
+
+def test_autotune_leaves_trajectory_unchanged():
+    banner("autotune")
+    spec = W.mixed_arch(scale=4)
+    g, o = pair(spec, seed=8)
+    g.grid_density()
+    g.update_omega()
+    lanes, ms = g.autotune(2)
+    assert lanes in (-1, 1, 2, 4, 8) and ms > 0
+    assert g.stats().accepts == 0 and g.stats().sweeps == 0
+    pos = o.init_coords()
+    om = g.omega()
+    g.mc_sweep(2)
+    o.sweep(pos, om, 0)
+    o.sweep(pos, om, 1)
+    assert same_bits(g.positions(), pos)
     g.close()
 
 
