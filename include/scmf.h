@@ -76,6 +76,9 @@ typedef struct scmf_opts {
                                staged in LDS; -1 = one lane per chain in global
                                memory; 0 = pick automatically.  Results do not
                                depend on it. */
+  int32_t sort_every;       /* reorder chains spatially every k sweeps (enables the
+                               tiled gridding); 0 = automatic, -1 = never.
+                               Results do not depend on it. */
 } scmf_opts;
 
 typedef struct scmf_stats_t {
@@ -83,7 +86,9 @@ typedef struct scmf_stats_t {
   uint64_t accepts;      /* accepted moves */
   uint64_t sweeps;       /* sweeps done (also the sweep word of the next sweep's stream) */
   double re2;            /* mean squared end-to-end distance, first to last bead (Re^2) */
-  int32_t overflow;      /* 1 if a uint16 count saturated since the last gridding */
+  int32_t overflow;      /* 1 if a uint16 count saturated since the last reset */
+  uint64_t grid_fallback; /* beads the tiled gridding counted with global atomics */
+  uint64_t sorts;         /* spatial reorderings done */
 } scmf_stats_t;
 
 typedef struct scmf_timing_t {
@@ -147,6 +152,10 @@ int scmf_copy_in_pos(scmf_sys* sys, const float* xyz, size_t n_beads);
 int scmf_copy_in_omega(scmf_sys* sys, const float* omega, size_t n_values);
 /* Density counts [types][cells] (uint32; must fit the system's count width). */
 int scmf_copy_in_density(scmf_sys* sys, const uint32_t* counts, size_t n_values);
+
+/* Reorder the chains spatially now (storage order only: results, copies and
+ * files are unaffected).  No counterpart in the reference. */
+int scmf_sort_chains(scmf_sys* sys);
 
 /* Re-key the random streams: seed, and the sweep counter of the next sweep. */
 int scmf_set_rng(scmf_sys* sys, uint64_t seed, uint64_t next_sweep);

@@ -51,6 +51,7 @@ class Opts(C.Structure):
         ("density_bits", C.c_int32),
         ("init_coords", C.c_int32),
         ("lanes_per_chain", C.c_int32),
+        ("sort_every", C.c_int32),
     ]
 
 
@@ -61,6 +62,8 @@ class Stats(C.Structure):
         ("sweeps", C.c_uint64),
         ("re2", C.c_double),
         ("overflow", C.c_int32),
+        ("grid_fallback", C.c_uint64),
+        ("sorts", C.c_uint64),
     ]
 
 
@@ -176,7 +179,7 @@ class SystemSpec:
 
 
 def make_opts(seed=1, system_id=0, move="smart", trial_delta=0.1, omega_every_step=False,
-              density_bits=32, init_coords=True, lanes_per_chain=0) -> Opts:
+              density_bits=32, init_coords=True, lanes_per_chain=0, sort_every=0) -> Opts:
     o = Opts()
     o.seed = int(seed)
     o.system_id = int(system_id)
@@ -186,4 +189,5 @@ def make_opts(seed=1, system_id=0, move="smart", trial_delta=0.1, omega_every_st
     o.density_bits = int(density_bits)
     o.init_coords = 1 if init_coords else 0
     o.lanes_per_chain = int(lanes_per_chain)
+    o.sort_every = int(sort_every)
     return o

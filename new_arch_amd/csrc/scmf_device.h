@@ -40,6 +40,9 @@ struct KParams {
   const int* monotype;           // [sum N_p]
   const int* nbr_start;          // [sum N_p + 1], CSR over every bead of every type
   const int* nbr;                // neighbour bead indices within the chain
+  const uint32_t* slot_chain;    // chain id stored in each slot (slots ordered spatially)
+  const uint32_t* chain_slot;    // slot of each chain id
+  const uint8_t* btype;          // monomer type of every stored bead
   int64_t n_chains, n_beads;
   uint64_t seed;
   uint32_t system_id;

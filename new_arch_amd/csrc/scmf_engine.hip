@@ -25,6 +25,7 @@
 #include "../../include/scmf.h"
 #include "scmf_device.h"
 #include "scmf_host.h"
+#include "scmf_tiles.h"
 
 #pragma clang fp contract(off)
 
@@ -230,8 +231,8 @@ __global__ void __launch_bounds__(BLOCK) k_sweep_global(KParams kp, float* __res
     const int length = kp.len[p];
     const int64_t off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
     const GlobalChain c{x + off, y + off, z + off};
-    for (int i = 0; i < length; i++)
-      acc += one_move(kp, c, p, length, (uint32_t)chain, sweep, (uint32_t)i, omega);
+    const uint32_t id = kp.slot_chain[chain];
+    for (int i = 0; i < length; i++) acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, omega);
   }
   wave_count(acc, accepts);
 }
@@ -262,8 +263,8 @@ __global__ void __launch_bounds__(64) k_sweep_lds(KParams kp, float* __restrict_
       sz[b * 64] = z[off + b];
     }
     const LdsChain<64> c{sx, sy, sz};
-    for (int i = 0; i < length; i++)
-      acc += one_move(kp, c, p, length, (uint32_t)chain, sweep, (uint32_t)i, omega);
+    const uint32_t id = kp.slot_chain[chain];
+    for (int i = 0; i < length; i++) acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, omega);
     for (int b = 0; b < length; ++b) {
       x[off + b] = sx[b * 64];
       y[off + b] = sy[b * 64];
@@ -312,10 +313,12 @@ __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restric
   const bool active = chain < kp.n_chains;
   int p = 0, length = 0;
   int64_t off = 0;
+  uint32_t id = 0;
   if (active) {
     p = arch_of_chain(kp, chain);
     length = kp.len[p];
     off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
+    id = kp.slot_chain[chain];
   }
   // load the tile and draw the picks
   for (int b = r; b < length; b += L) {
@@ -324,8 +327,7 @@ __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restric
     sz[b * G + slot] = z[off + b];
   }
   for (int i = r; i < length; i += L) {
-    const u32x4 pick = stream_block(kp.seed, kp.system_id, (uint32_t)chain, sweep, (uint32_t)i,
-                                    SLOT_PICK);
+    const u32x4 pick = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i, SLOT_PICK);
     int ib = (int)(u01_53(pick.v[0], pick.v[1]) * length);
     if (ib == length) ib = length - 1;
     lev[i] = (I)ib;
@@ -374,7 +376,7 @@ __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restric
   for (int st = 0; st < wsteps; ++st) {
     const int i = active && st * L < length * L ? (int)sched[st * L + r] : (int)EMPTY;
     if (i != (int)EMPTY)
-      acc += one_move(kp, c, p, length, (uint32_t)chain, sweep, (uint32_t)i, omega);
+      acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, omega);
     __syncthreads();
   }
   for (int b = r; b < length; b += L) {
@@ -405,15 +407,7 @@ __global__ void __launch_bounds__(BLOCK) k_grid(KParams kp, const float* __restr
                                                 uint32_t* __restrict__ overflow) {
   const int64_t stride = (int64_t)gridDim.x * BLOCK;
   for (int64_t g = (int64_t)blockIdx.x * BLOCK + threadIdx.x; g < kp.n_beads; g += stride) {
-    const int p = arch_of_bead(kp, g);
-    // monomer index rel mod len: quotient from one fp64 product, which is
-    // at most one below the true quotient; the correction makes it exact
-    const uint32_t rel = (uint32_t)(g - kp.bead_base[p]);
-    const uint32_t q = (uint32_t)((double)rel * kp.inv_len[p]);
-    int m = (int)(rel - q * (uint32_t)kp.len[p]);
-    if (m >= kp.len[p]) m -= kp.len[p];
-    if (m < 0) m += kp.len[p];
-    const uint32_t t = (uint32_t)kp.monotype[kp.mono_off[p] + m];
+    const uint32_t t = kp.btype[g];
     const uint32_t cell = bin_cell(kp, (double)x[g], (double)y[g], (double)z[g], t);
     if (BITS == 32) {
       atomicAdd(&words[cell], 1u);
@@ -470,7 +464,8 @@ __global__ void __launch_bounds__(BLOCK) k_re2(KParams kp, const float* __restri
   double v = 0;
   if (chain < kp.n_chains) {
     const int p = arch_of_chain(kp, chain);
-    const int64_t a = kp.bead_base[p] + (chain - kp.chain_base[p]) * kp.len[p];
+    const int64_t slot = kp.chain_slot[chain];
+    const int64_t a = kp.bead_base[p] + (slot - kp.chain_base[p]) * kp.len[p];
     const int64_t e = a + kp.len[p] - 1;
     const double ddx = (double)x[a] - (double)x[e];
     const double ddy = (double)y[a] - (double)y[e];
@@ -486,28 +481,72 @@ __global__ void __launch_bounds__(BLOCK) k_re2(KParams kp, const float* __restri
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ void __launch_bounds__(BLOCK) k_planar_to_aos(const float* __restrict__ x,
-                                                         const float* __restrict__ y,
-                                                         const float* __restrict__ z,
-                                                         float* __restrict__ aos, int64_t n) {
+// Stored index of reference bead r (reference order: chains by id, beads in
+// order): the chain's slot replaces its id inside its architecture block.
+__device__ __forceinline__ int64_t stored_index(const KParams& kp, int64_t r) {
+  const int p = arch_of_bead(kp, r);
+  const int len = kp.len[p];
+  const int64_t rel = r - kp.bead_base[p];
+  int64_t q = (int64_t)((double)rel * kp.inv_len[p]);
+  int64_t m = rel - q * len;
+  if (m >= len) {
+    m -= len;
+    ++q;
+  }
+  if (m < 0) {
+    m += len;
+    --q;
+  }
+  const int64_t slot = kp.chain_slot[kp.chain_base[p] + q];
+  return kp.bead_base[p] + (slot - kp.chain_base[p]) * len + m;
+}
+
+// Reference beads [r0, r0 + n) to float [bead][xyz].
+__global__ void __launch_bounds__(BLOCK) k_gather_ref(KParams kp, const float* __restrict__ x,
+                                                      const float* __restrict__ y,
+                                                      const float* __restrict__ z,
+                                                      float* __restrict__ aos, int64_t r0,
+                                                      int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * BLOCK;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
-    aos[3 * i] = x[i];
-    aos[3 * i + 1] = y[i];
-    aos[3 * i + 2] = z[i];
+    const int64_t src = stored_index(kp, r0 + i);
+    aos[3 * i] = x[src];
+    aos[3 * i + 1] = y[src];
+    aos[3 * i + 2] = z[src];
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_aos_to_planar(const float* __restrict__ aos,
-                                                         float* __restrict__ x,
-                                                         float* __restrict__ y,
-                                                         float* __restrict__ z, int64_t n) {
+__global__ void __launch_bounds__(BLOCK) k_scatter_ref(KParams kp, const float* __restrict__ aos,
+                                                       float* __restrict__ x,
+                                                       float* __restrict__ y,
+                                                       float* __restrict__ z, int64_t r0,
+                                                       int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * BLOCK;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
-    x[i] = aos[3 * i];
-    y[i] = aos[3 * i + 1];
-    z[i] = aos[3 * i + 2];
+    const int64_t dst = stored_index(kp, r0 + i);
+    x[dst] = aos[3 * i];
+    y[dst] = aos[3 * i + 1];
+    z[dst] = aos[3 * i + 2];
   }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_fill_btype(KParams kp, uint8_t* __restrict__ bt) {
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t g = (int64_t)blockIdx.x * BLOCK + threadIdx.x; g < kp.n_beads; g += stride) {
+    const int p = arch_of_bead(kp, g);
+    const int len = kp.len[p];
+    const int64_t rel = g - kp.bead_base[p];
+    int64_t q = (int64_t)((double)rel * kp.inv_len[p]);
+    int64_t m = rel - q * len;
+    if (m >= len) m -= len;
+    if (m < 0) m += len;
+    bt[g] = (uint8_t)kp.monotype[kp.mono_off[p] + m];
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_iota(uint32_t* __restrict__ a, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) a[i] = (uint32_t)i;
 }
 
 template <int BITS>
@@ -577,6 +616,26 @@ struct scmf_sys {
   int64_t n_beads = 0, n_chains = 0, n_cells = 0;
   int T = 0;
   int nmax = 0;         // longest chain
+  // spatial ordering of the slots and the tiled gridding
+  uint32_t* slot_chain = nullptr;
+  uint32_t* chain_slot = nullptr;
+  uint32_t* slot_chain2 = nullptr;
+  uint8_t* btype = nullptr;
+  uint32_t* keys = nullptr;
+  uint32_t* tcount = nullptr;
+  uint32_t* tstart = nullptr;
+  uint32_t* tfill = nullptr;
+  uint32_t* new_slot = nullptr;
+  float* x2 = nullptr;
+  float* y2 = nullptr;
+  float* z2 = nullptr;
+  uint32_t* scratch = nullptr;
+  unsigned long long* d_fallback = nullptr;
+  TileGeo geo{};
+  size_t tile_lds = 0;  // LDS of k_grid_tiles; 0 = tiled gridding not usable
+  bool sorted = false;
+  int sort_every = 0;
+  uint64_t n_sorts = 0;
   size_t lds_bytes = 0; // LDS of the sweep kernel
   const SweepVariant* variant = nullptr;
   float* x = nullptr;
@@ -833,11 +892,128 @@ int launch_sweep(scmf_sys* s) {
 
 
 
+// Tile geometry of the tiled gridding.  The halo covers the reach of a chain
+// from its middle bead (0.75 Re sqrt((N-1)/(Nref-1)), in cells).  Per axis the
+// tile edge ts is the largest that keeps the window (ts + 2h per axis) inside
+// the LDS budget and the grid, with every tile (the last, partial one
+// included) at least h cells long, so a cell is covered only by the windows of
+// its own tile and the two next to it.  The budget is the one maximising
+// interior/window volume times resident workgroups per CU.
+void choose_tiles(scmf_sys* s) {
+  const KParams& kp = s->kp;
+  s->tile_lds = 0;
+  const double dmin = std::min(kp.dx, std::min(kp.dy, kp.dz));
+  const double reach = 0.75 * sqrt((double)std::max(s->nmax - 1, 1) / (double)(s->desc.nref - 1));
+  int h = (int)ceil(reach / dmin);
+  h = std::max(2, std::min(h, 16));
+  const int n[3] = {kp.nx, kp.ny, kp.nz};
+  double best = 0;
+  TileGeo bg{};
+  size_t bbytes = 0;
+  for (size_t budget : {(size_t)64 << 10, (size_t)96 << 10, (size_t)144 << 10}) {
+    int Wc = 1;  // cube edge the budget allows
+    while ((size_t)(Wc + 1) * (Wc + 1) * (Wc + 1) * s->T * 2 <= budget) ++Wc;
+    TileGeo g{};
+    g.h = h;
+    bool ok = true;
+    for (int a = 0; a < 3 && ok; ++a) {
+      int ts = std::min(Wc, n[a]) - 2 * h;
+      while (ts >= h && n[a] % ts != 0 && n[a] % ts < h) --ts;
+      if (ts < h || ts < 2) ok = false;
+      g.ts[a] = ts;
+      g.W[a] = ts + 2 * h;
+    }
+    if (!ok) continue;
+    const double frac = (double)g.ts[0] * g.ts[1] * g.ts[2] / ((double)g.W[0] * g.W[1] * g.W[2]);
+    const size_t bytes = (((size_t)g.W[0] * g.W[1] * g.W[2] * s->T + 1) / 2) * 4;
+    const int per_cu = (int)std::max<size_t>(1, ((size_t)160 << 10) / bytes);
+    const double score = frac * std::min(per_cu, 2);
+    if (score > best) {
+      best = score;
+      bbytes = bytes;
+      bg = g;
+    }
+  }
+  if (best <= 0) return;
+  for (int a = 0; a < 3; ++a) bg.nt[a] = (n[a] + bg.ts[a] - 1) / bg.ts[a];
+  bg.ntiles = bg.nt[0] * bg.nt[1] * bg.nt[2];
+  bg.win_cells = bg.W[0] * bg.W[1] * bg.W[2];
+  bg.win_words = (int)(((int64_t)bg.win_cells * s->T + 1) / 2);
+  s->geo = bg;
+  s->tile_lds = bbytes;
+}
+
+int alloc_sort_buffers(scmf_sys* s) {
+  if (s->keys) return SCMF_OK;
+  const int64_t C = s->n_chains, B = s->n_beads;
+  const int64_t nk = (int64_t)s->kp.P * s->geo.ntiles;
+  auto A = [&](void** p, size_t bytes) -> int {
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(SCMF_E_NOMEM, "device allocation of " + std::to_string(bytes) +
+                                    " bytes for the spatial ordering failed");
+    }
+    return SCMF_OK;
+  };
+  int rc;
+  if ((rc = A((void**)&s->keys, sizeof(uint32_t) * C)) ||
+      (rc = A((void**)&s->new_slot, sizeof(uint32_t) * C)) ||
+      (rc = A((void**)&s->slot_chain2, sizeof(uint32_t) * C)) ||
+      (rc = A((void**)&s->tcount, sizeof(uint32_t) * nk)) ||
+      (rc = A((void**)&s->tfill, sizeof(uint32_t) * nk)) ||
+      (rc = A((void**)&s->tstart, sizeof(uint32_t) * (nk + 1))) ||
+      (rc = A((void**)&s->x2, sizeof(float) * B)) || (rc = A((void**)&s->y2, sizeof(float) * B)) ||
+      (rc = A((void**)&s->z2, sizeof(float) * B)) ||
+      (rc = A((void**)&s->scratch, sizeof(uint32_t) * (size_t)s->geo.ntiles * s->geo.win_words)))
+    return rc;
+  if (s->tile_lds > 65536 &&
+      (hipFuncSetAttribute((const void*)k_grid_tiles<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)s->tile_lds) != hipSuccess ||
+       hipFuncSetAttribute((const void*)k_grid_tiles<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)s->tile_lds) != hipSuccess)) {
+    (void)hipGetLastError();
+    return fail(SCMF_E_HIP, "cannot raise the LDS limit of the tiled gridding");
+  }
+  return SCMF_OK;
+}
+
+// Reorder the slots by (architecture, tile of the middle bead).
+int sort_chains(scmf_sys* s) {
+  if (!s->tile_lds) return SCMF_OK;
+  int rc = alloc_sort_buffers(s);
+  if (rc) return rc;
+  const int64_t C = s->n_chains;
+  const int nk = s->kp.P * s->geo.ntiles;
+  hipLaunchKernelGGL(k_zero, dim3(grid_strided(nk)), dim3(BLOCK), 0, s->stream, s->tcount,
+                     (int64_t)nk);
+  hipLaunchKernelGGL(k_tile_keys, dim3(grid_for(C)), dim3(256), 0, s->stream, s->kp, s->geo, s->x,
+                     s->y, s->z, s->keys, s->tcount);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s->stream, s->tcount, s->tstart,
+                     s->tfill, nk);
+  hipLaunchKernelGGL(k_tile_scatter, dim3(grid_for(C)), dim3(256), 0, s->stream, C, s->keys,
+                     s->tstart, s->tfill, s->new_slot);
+  hipLaunchKernelGGL(k_permute, dim3(grid_strided(s->n_beads)), dim3(256), 0, s->stream, s->kp,
+                     s->x, s->y, s->z, s->x2, s->y2, s->z2, s->new_slot, s->slot_chain,
+                     s->slot_chain2, s->chain_slot);
+  HIP_TRY(hipGetLastError());
+  std::swap(s->x, s->x2);
+  std::swap(s->y, s->y2);
+  std::swap(s->z, s->z2);
+  std::swap(s->slot_chain, s->slot_chain2);
+  s->kp.slot_chain = s->slot_chain;
+  s->sorted = true;
+  s->n_sorts += 1;
+  return SCMF_OK;
+}
+
 int free_all(scmf_sys* s) {
   for (void* p : {(void*)s->x, (void*)s->y, (void*)s->z, (void*)s->counts, (void*)s->omega,
                   (void*)s->d_monotype, (void*)s->d_nbr_start, (void*)s->d_nbr,
                   (void*)s->d_accepts, (void*)s->d_overflow, (void*)s->d_partial,
-                  (void*)s->staging})
+                  (void*)s->staging, (void*)s->slot_chain, (void*)s->chain_slot,
+                  (void*)s->slot_chain2, (void*)s->btype, (void*)s->keys, (void*)s->tcount,
+                  (void*)s->tstart, (void*)s->tfill, (void*)s->new_slot, (void*)s->x2,
+                  (void*)s->y2, (void*)s->z2, (void*)s->scratch, (void*)s->d_fallback})
     if (p) (void)hipFree(p);
   for (Timed& t : s->pending) {
     (void)hipEventDestroy(t.a);
@@ -908,7 +1084,24 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
       (rc = alloc((void**)&s->d_overflow, sizeof(uint32_t))) ||
       (rc = alloc((void**)&s->d_partial, sizeof(double) * grid_for(s->n_chains))))
     return bail(rc);
-  if (hipMemsetAsync(s->d_accepts, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
+  if ((rc = alloc((void**)&s->slot_chain, sizeof(uint32_t) * s->n_chains)) ||
+      (rc = alloc((void**)&s->chain_slot, sizeof(uint32_t) * s->n_chains)) ||
+      (rc = alloc((void**)&s->btype, (size_t)B)) ||
+      (rc = alloc((void**)&s->d_fallback, sizeof(unsigned long long))))
+    return bail(rc);
+  s->kp.slot_chain = s->slot_chain;
+  s->kp.chain_slot = s->chain_slot;
+  s->kp.btype = s->btype;
+  hipLaunchKernelGGL(k_iota, dim3(grid_strided(s->n_chains)), dim3(BLOCK), 0, s->stream,
+                     s->slot_chain, s->n_chains);
+  hipLaunchKernelGGL(k_iota, dim3(grid_strided(s->n_chains)), dim3(BLOCK), 0, s->stream,
+                     s->chain_slot, s->n_chains);
+  hipLaunchKernelGGL(k_fill_btype, dim3(grid_strided(B)), dim3(BLOCK), 0, s->stream, s->kp, s->btype);
+  choose_tiles(s);
+  s->sort_every = o.sort_every == 0 ? (s->tile_lds ? 25 : -1) : o.sort_every;
+  if (s->sort_every < 0) s->tile_lds = 0;
+  if (hipMemsetAsync(s->d_fallback, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
+      hipMemsetAsync(s->d_accepts, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
       hipMemsetAsync(s->d_overflow, 0, sizeof(uint32_t), s->stream) != hipSuccess ||
       hipMemsetAsync(s->omega, 0, sizeof(float) * TC, s->stream) != hipSuccess ||
       hipMemsetAsync(s->counts, 0, sizeof(uint32_t) * s->count_words, s->stream) != hipSuccess)
@@ -1004,6 +1197,10 @@ int scmf_mc_sweep(scmf_sys* s, int32_t n_sweeps) {
 
 int scmf_grid_density(scmf_sys* s) {
   if (!s) return fail(SCMF_E_ARG, "null handle");
+  if (s->tile_lds && !s->sorted) {
+    int rc = sort_chains(s);
+    if (rc) return rc;
+  }
   Timed t;
   if (s->timing) {
     int rc = s->ev_begin(K_GRID, &t);
@@ -1011,15 +1208,40 @@ int scmf_grid_density(scmf_sys* s) {
   }
   hipLaunchKernelGGL(k_zero, dim3(grid_strided(s->count_words)), dim3(BLOCK), 0, s->stream,
                      s->counts, s->count_words);
-  if (s->opts.density_bits == 32)
+  const bool b32 = s->opts.density_bits == 32;
+  if (s->tile_lds) {
+    const dim3 tiles((unsigned)s->geo.ntiles);
+    if (b32)
+      hipLaunchKernelGGL(k_grid_tiles<32>, tiles, dim3(1024), s->tile_lds, s->stream, s->kp,
+                         s->geo, s->x, s->y, s->z, s->btype, s->tstart, s->scratch, s->counts,
+                         s->d_overflow, s->d_fallback);
+    else
+      hipLaunchKernelGGL(k_grid_tiles<16>, tiles, dim3(1024), s->tile_lds, s->stream, s->kp,
+                         s->geo, s->x, s->y, s->z, s->btype, s->tstart, s->scratch, s->counts,
+                         s->d_overflow, s->d_fallback);
+    if (b32)
+      hipLaunchKernelGGL(k_tiles_sum<32>, dim3(grid_strided(s->count_words)), dim3(256), 0,
+                         s->stream, s->kp, s->geo, s->scratch, s->counts, s->d_overflow,
+                         s->count_words);
+    else
+      hipLaunchKernelGGL(k_tiles_sum<16>, dim3(grid_strided(s->count_words)), dim3(256), 0,
+                         s->stream, s->kp, s->geo, s->scratch, s->counts, s->d_overflow,
+                         s->count_words);
+  } else if (b32) {
     hipLaunchKernelGGL(k_grid<32>, dim3(grid_strided(s->n_beads)), dim3(BLOCK), 0, s->stream,
                        s->kp, s->x, s->y, s->z, s->counts, s->d_overflow);
-  else
+  } else {
     hipLaunchKernelGGL(k_grid<16>, dim3(grid_strided(s->n_beads)), dim3(BLOCK), 0, s->stream,
                        s->kp, s->x, s->y, s->z, s->counts, s->d_overflow);
+  }
   HIP_TRY(hipGetLastError());
   if (s->timing) return s->ev_end(&t);
   return SCMF_OK;
+}
+
+int scmf_sort_chains(scmf_sys* s) {
+  if (!s) return fail(SCMF_E_ARG, "null handle");
+  return sort_chains(s);
 }
 
 int scmf_update_omega(scmf_sys* s) {
@@ -1041,6 +1263,11 @@ int scmf_update_omega(scmf_sys* s) {
 }
 
 int scmf_step(scmf_sys* s, int32_t step) {
+  if (!s) return fail(SCMF_E_ARG, "null handle");
+  if (s->tile_lds && s->sort_every > 0 && s->sweeps % (uint64_t)s->sort_every == 0) {
+    int rc = sort_chains(s);
+    if (rc) return rc;
+  }
   int rc = scmf_mc_sweep(s, 1);
   if (rc) return rc;
   if ((rc = scmf_grid_density(s))) return rc;
@@ -1057,8 +1284,8 @@ int scmf_copy_out(scmf_sys* s, int32_t what, void* host, size_t bytes) {
     if (rc) return rc;
     for (int64_t b0 = 0; b0 < s->n_beads; b0 += STAGE_BEADS) {
       const int64_t n = std::min<int64_t>(STAGE_BEADS, s->n_beads - b0);
-      hipLaunchKernelGGL(k_planar_to_aos, dim3(grid_strided(n)), dim3(BLOCK), 0, s->stream,
-                         s->x + b0, s->y + b0, s->z + b0, s->staging, n);
+      hipLaunchKernelGGL(k_gather_ref, dim3(grid_strided(n)), dim3(BLOCK), 0, s->stream, s->kp,
+                         s->x, s->y, s->z, s->staging, b0, n);
       HIP_TRY(hipGetLastError());
       HIP_TRY(hipMemcpyAsync((float*)host + 3 * b0, s->staging, sizeof(float) * 3 * n,
                              hipMemcpyDeviceToHost, s->stream));
@@ -1102,8 +1329,8 @@ int scmf_copy_in_pos(scmf_sys* s, const float* xyz, size_t n_beads) {
     const int64_t n = std::min<int64_t>(STAGE_BEADS, s->n_beads - b0);
     HIP_TRY(hipMemcpyAsync(s->staging, xyz + 3 * b0, sizeof(float) * 3 * n, hipMemcpyHostToDevice,
                            s->stream));
-    hipLaunchKernelGGL(k_aos_to_planar, dim3(grid_strided(n)), dim3(BLOCK), 0, s->stream,
-                       s->staging, s->x + b0, s->y + b0, s->z + b0, n);
+    hipLaunchKernelGGL(k_scatter_ref, dim3(grid_strided(n)), dim3(BLOCK), 0, s->stream, s->kp,
+                       s->staging, s->x, s->y, s->z, b0, n);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(s->stream));
   }
@@ -1154,11 +1381,12 @@ int scmf_stats(scmf_sys* s, scmf_stats_t* out) {
                      s->d_partial);
   HIP_TRY(hipGetLastError());
   std::vector<double> part(nb);
-  unsigned long long acc = 0;
+  unsigned long long acc = 0, fb = 0;
   uint32_t ovf = 0;
   HIP_TRY(hipMemcpyAsync(part.data(), s->d_partial, sizeof(double) * nb, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipMemcpyAsync(&acc, s->d_accepts, sizeof acc, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipMemcpyAsync(&ovf, s->d_overflow, sizeof ovf, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipMemcpyAsync(&fb, s->d_fallback, sizeof fb, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   double sum = 0;
   for (double v : part) sum += v;
@@ -1167,6 +1395,8 @@ int scmf_stats(scmf_sys* s, scmf_stats_t* out) {
   out->sweeps = s->sweeps;
   out->re2 = sum / (double)s->n_chains;
   out->overflow = ovf ? 1 : 0;
+  out->grid_fallback = fb;
+  out->sorts = s->n_sorts;
   return SCMF_OK;
 }
 
@@ -1174,6 +1404,7 @@ int scmf_reset_stats(scmf_sys* s) {
   if (!s) return fail(SCMF_E_ARG, "null handle");
   HIP_TRY(hipMemsetAsync(s->d_accepts, 0, sizeof(unsigned long long), s->stream));
   HIP_TRY(hipMemsetAsync(s->d_overflow, 0, sizeof(uint32_t), s->stream));
+  HIP_TRY(hipMemsetAsync(s->d_fallback, 0, sizeof(unsigned long long), s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->moves = 0;
   return SCMF_OK;

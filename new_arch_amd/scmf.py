@@ -39,7 +39,7 @@ def lib():
     for name in ("scmf_mc_sweep", "scmf_step", "scmf_enable_timing"):
         getattr(L, name).argtypes = [vp, C.c_int32]
     for name in ("scmf_grid_density", "scmf_update_omega", "scmf_reset_stats",
-                 "scmf_reset_timing", "scmf_sync"):
+                 "scmf_reset_timing", "scmf_sync", "scmf_sort_chains"):
         getattr(L, name).argtypes = [vp]
     L.scmf_copy_out.argtypes = [vp, C.c_int32, vp, C.c_size_t]
     L.scmf_copy_in_pos.argtypes = [vp, vp, C.c_size_t]
@@ -142,6 +142,9 @@ class System:
 
     def sync(self):
         _check(lib().scmf_sync(self._h))
+
+    def sort_chains(self):
+        _check(lib().scmf_sort_chains(self._h))
 
     # -- data movement
     def positions(self) -> np.ndarray:

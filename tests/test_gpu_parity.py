@@ -194,6 +194,72 @@ def test_autotune_leaves_trajectory_unchanged():
     g.close()
 
 
+def tiled_spec(grid, T=2, N=16):
+    L = [g * 0.125 for g in grid]
+    poly = [PolymerType(length=N, count=600, monotype=[t % T for t in range(N)])]
+    if T == 3:
+        poly.append(PolymerType(length=N // 2, count=400, monotype=[2] * (N // 2)))
+    return SystemSpec(n_types=T, xn=W.xn_matrix(T, 40.0, 15.0), L=L, n=list(grid), dtNxi=0.1,
+                      nref=N, polymers=poly)
+
+
+@pytest.mark.parametrize("bits", [16, 32])
+@pytest.mark.parametrize("grid,T", [((27, 31, 40), 2), ((32, 32, 32), 3), ((64, 24, 33), 2)])
+def test_tiled_gridding_bit_exact(grid, T, bits):
+    """Spatially ordered chains, windows with partial tiles, outliers on the
+    global-atomic fallback; counts equal the oracle's at every stage."""
+    banner(f"tiled grid {grid} T={T} u{bits}")
+    spec = tiled_spec(grid, T)
+    g, o = pair(spec, seed=5, density_bits=bits, sort_every=3)
+    pos = o.init_coords()
+    g.grid_density()                               # orders the chains first
+    np.testing.assert_array_equal(g.density(), o.grid(pos))
+    st = g.stats()
+    assert st.sorts == 1
+    assert same_bits(g.positions(), pos)            # storage order is invisible
+    # outliers: a few chains spread over far periodic images
+    rng = np.random.default_rng(1)
+    far = pos.copy()
+    idx = rng.choice(spec.n_beads, spec.n_beads // 20, replace=False)
+    far[idx] += (rng.integers(-3, 4, (idx.size, 3)) * np.array(spec.L)
+                 + rng.uniform(-2, 2, (idx.size, 3))).astype(np.float32)
+    g.set_positions(far)
+    g.reset_stats()
+    g.grid_density()
+    np.testing.assert_array_equal(g.density(), o.grid(far))
+    assert g.stats().grid_fallback > 0
+    # steps with reordering every 3 sweeps
+    g.update_omega()
+    om = g.omega()
+    for step in range(1, 8):
+        g.step(step)
+        o.sweep(far, om, step - 1)
+        counts = o.grid(far)
+        om = o.omega(counts).astype(np.float32) if step % 10 else om
+    np.testing.assert_array_equal(g.density(), counts)
+    assert same_bits(g.positions(), far)
+    assert g.stats().sorts >= 3
+    g.close()
+
+
+def test_sort_is_invisible():
+    banner("sort invisible")
+    spec = W.small_diblock_melt()
+    g, o = pair(spec, seed=2, sort_every=-1)
+    pos = o.init_coords()
+    re2 = g.stats().re2
+    g.sort_chains()
+    assert same_bits(g.positions(), pos)
+    assert g.stats().re2 == re2
+    g.grid_density()
+    g.update_omega()
+    om = g.omega()
+    g.mc_sweep(1)
+    o.sweep(pos, om, 0)
+    assert same_bits(g.positions(), pos)
+    g.close()
+
+
 def test_set_rng_replays_a_sweep():
     banner("set_rng replay")
     spec = W.tiny()
