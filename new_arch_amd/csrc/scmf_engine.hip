@@ -156,9 +156,8 @@ __device__ __forceinline__ uint32_t one_move(const KParams& kp, const V& c, int 
     const float R = kp.R[iwtype];
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     bond_force(kp, c, p, ibead, length, px, py, pz, &fx, &fy, &fz);
-    double n0, n1, n2;
-    normal3(disp, &n0, &n1, &n2);
-    const float rx = (float)n0, ry = (float)n1, rz = (float)n2;
+    float rx, ry, rz;
+    normal3f(disp, &rx, &ry, &rz);
     dx = A * fx + rx * R;
     dy = A * fy + ry * R;
     dz = A * fz + rz * R;
@@ -286,8 +285,10 @@ __global__ void __launch_bounds__(64) k_sweep_lds(KParams kp, float* __restrict_
 // exactly the state it sees in the sequential order, so results are
 // bit-identical to one lane per chain; the per-wavefront LDS tile shrinks by L.
 //
-// LDS per chain (index type I: uint8 for N <= 255):
-//   positions 3 N floats (bead b at [b*G + slot]), lev N, W N+1, cnt N+1, sched N*L
+// LDS per chain (index type I: uint8 for N <= 255): positions 3N floats
+// (bead b at [b*G + slot]); lev[N] (pick, then level of each move);
+// W[N+1] (level per bead during the scan, then per-level fill counts);
+// start[N+2] (first list entry of each level); list[N] (moves by level).
 template <int L, typename I>
 __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restrict__ x,
                                                     float* __restrict__ y,
@@ -296,7 +297,6 @@ __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restric
                                                     uint32_t sweep, int nmax,
                                                     unsigned long long* __restrict__ accepts) {
   constexpr int G = 64 / L;
-  constexpr I EMPTY = (I)~(I)0;
   extern __shared__ float lds[];
   const int lane = threadIdx.x;
   const int slot = lane / L, r = lane % L;
@@ -304,11 +304,11 @@ __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restric
   float* sy = lds + nmax * G;
   float* sz = lds + 2 * nmax * G;
   I* book = (I*)(lds + 3 * nmax * G);
-  const int per = 3 * nmax + 2 + nmax * L;  // lev, W, cnt, sched
+  const int per = 4 * nmax + 3;
   I* lev = book + slot * per;
   I* W = lev + nmax;
-  I* cnt = W + nmax + 1;
-  I* sched = cnt + nmax + 1;
+  I* start = W + nmax + 1;
+  I* list = start + nmax + 2;
   const int64_t chain = (int64_t)blockIdx.x * G + slot;
   const bool active = chain < kp.n_chains;
   int p = 0, length = 0;
@@ -331,17 +331,11 @@ __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restric
     int ib = (int)(u01_53(pick.v[0], pick.v[1]) * length);
     if (ib == length) ib = length - 1;
     lev[i] = (I)ib;
-    W[i] = 0;
-    cnt[i] = 0;
   }
-  for (int k = r; k < length * L; k += L) sched[k] = EMPTY;
-  if (r == 0) {
-    cnt[length] = 0;
-    W[length] = 0;
-  }
+  for (int k = r; k <= length; k += L) W[k] = 0;
+  for (int k = r; k <= length + 1; k += L) start[k] = 0;
   __syncthreads();
-  // dependency levels (one lane per chain), then step offsets and the schedule
-  int steps = 0;
+  // dependency levels (one lane per chain), level sizes, moves listed by level
   if (r == 0 && active) {
     int maxl = 0;
     for (int i = 0; i < length; ++i) {
@@ -351,32 +345,36 @@ __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restric
       l += 1;
       W[b] = (I)l;
       lev[i] = (I)l;
-      cnt[l] = (I)(cnt[l] + 1);
+      start[l + 1] = (I)(start[l + 1] + 1);
       maxl = max(maxl, l);
     }
-    for (int l = 1; l <= maxl; ++l) {  // cnt[l] -> first step of level l
-      const int c = cnt[l];
-      cnt[l] = (I)steps;
-      steps += (c + L - 1) / L;
-      W[l] = 0;                        // W reused as the per-level fill counter
+    for (int l = 1; l <= maxl; ++l) {
+      start[l + 1] = (I)(start[l + 1] + start[l]);
+      W[l] = 0;
     }
     for (int i = 0; i < length; ++i) {
       const int l = lev[i];
       const int k = W[l];
       W[l] = (I)(k + 1);
-      sched[((int)cnt[l] + k / L) * L + k % L] = (I)i;
+      list[start[l] + k] = (I)i;
     }
+    W[0] = (I)maxl;
   }
-  // all lanes of the wavefront run max(steps) steps
-  int wsteps = steps;
-  for (int o = 32; o > 0; o >>= 1) wsteps = max(wsteps, __shfl_xor(wsteps, o, 64));
   __syncthreads();
+  const int maxl = active ? (int)W[0] : 0;
   uint32_t acc = 0;
   const LdsChain<G> c{sx + slot, sy + slot, sz + slot};
-  for (int st = 0; st < wsteps; ++st) {
-    const int i = active && st * L < length * L ? (int)sched[st * L + r] : (int)EMPTY;
-    if (i != (int)EMPTY)
-      acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, omega);
+  int l = 1, j = 0;
+  while (__any(l <= maxl)) {
+    if (l <= maxl) {
+      const int s0 = start[l], cl = (int)start[l + 1] - s0;
+      const int k = j * L + r;
+      if (k < cl) acc += one_move(kp, c, p, length, id, sweep, (uint32_t)list[s0 + k], omega);
+      if (++j * L >= cl) {
+        ++l;
+        j = 0;
+      }
+    }
     __syncthreads();
   }
   for (int b = r; b < length; b += L) {
@@ -390,7 +388,7 @@ __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restric
 template <int L, typename I>
 size_t sched_lds_bytes(int nmax) {
   constexpr int G = 64 / L;
-  return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (3 * nmax + 2 + nmax * L) * sizeof(I);
+  return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (4 * nmax + 3) * sizeof(I);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_zero(uint32_t* __restrict__ p, int64_t n) {
@@ -1099,7 +1097,6 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
   hipLaunchKernelGGL(k_fill_btype, dim3(grid_strided(B)), dim3(BLOCK), 0, s->stream, s->kp, s->btype);
   choose_tiles(s);
   s->sort_every = o.sort_every == 0 ? (s->tile_lds ? 25 : -1) : o.sort_every;
-  if (s->sort_every < 0) s->tile_lds = 0;
   if (hipMemsetAsync(s->d_fallback, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
       hipMemsetAsync(s->d_accepts, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
       hipMemsetAsync(s->d_overflow, 0, sizeof(uint32_t), s->stream) != hipSuccess ||
@@ -1197,7 +1194,7 @@ int scmf_mc_sweep(scmf_sys* s, int32_t n_sweeps) {
 
 int scmf_grid_density(scmf_sys* s) {
   if (!s) return fail(SCMF_E_ARG, "null handle");
-  if (s->tile_lds && !s->sorted) {
+  if (s->tile_lds && !s->sorted && s->sort_every > 0) {
     int rc = sort_chains(s);
     if (rc) return rc;
   }
@@ -1209,7 +1206,7 @@ int scmf_grid_density(scmf_sys* s) {
   hipLaunchKernelGGL(k_zero, dim3(grid_strided(s->count_words)), dim3(BLOCK), 0, s->stream,
                      s->counts, s->count_words);
   const bool b32 = s->opts.density_bits == 32;
-  if (s->tile_lds) {
+  if (s->tile_lds && s->sorted) {
     const dim3 tiles((unsigned)s->geo.ntiles);
     if (b32)
       hipLaunchKernelGGL(k_grid_tiles<32>, tiles, dim3(1024), s->tile_lds, s->stream, s->kp,

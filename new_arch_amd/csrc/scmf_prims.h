@@ -202,6 +202,96 @@ SCMF_HD void normal3(const u32x4& w, double* n0, double* n1, double* n2) {
   *n2 = r1 * c1;
 }
 
+// ---------------------------------------------------------------- fp32 normals
+// Natural log in fp32 for x in (0, 1] built like log_pos: x = 2^e m,
+// m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s), s = (m-1)/(m+1), odd series
+// to s^11 (truncation below 2^-30 relative).  LN2 split so e*LN2F_HI is exact.
+SCMF_HD float logf_pos(float x) {
+  union {
+    float f;
+    uint32_t u;
+  } bits;
+  bits.f = x;
+  int e = (int)((bits.u >> 23) & 0xFF) - 127;
+  bits.u = (bits.u & 0x007FFFFFu) | 0x3F800000u;
+  float m = bits.f;
+  if (m > 1.41421356f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  const float f = m - 1.0f;
+  const float s = f / (m + 1.0f);
+  const float z = s * s;
+  float p = 1.0f / 11.0f;
+  p = __builtin_fmaf(p, z, 1.0f / 9.0f);
+  p = __builtin_fmaf(p, z, 1.0f / 7.0f);
+  p = __builtin_fmaf(p, z, 1.0f / 5.0f);
+  p = __builtin_fmaf(p, z, 1.0f / 3.0f);
+  const float two_s = 2.0f * s;
+  const float logm = __builtin_fmaf(two_s * z, p, two_s);
+  const float LN2F_HI = 0x1.62e4p-1f;             // ln 2 with its low 8 bits cleared
+  const float LN2F_LO = 0x1.7f7d1cp-20f;          // ln 2 - LN2F_HI
+  const float fe = (float)e;
+  return __builtin_fmaf(fe, LN2F_HI, __builtin_fmaf(fe, LN2F_LO, logm));
+}
+
+// sin and cos of 2 pi k 2^-24 for a 24-bit integer k: the nearest quarter
+// turn is split off in integer arithmetic (exact), the rest, |t| <= pi/4,
+// goes through Taylor series to t^11 (sin) and t^10 (cos).
+SCMF_HD void sincosf_turn24(uint32_t k, float* s_out, float* c_out) {
+  const uint32_t q = (k + (1u << 21)) >> 22;                   // 0..4
+  const int32_t v = (int32_t)k - (int32_t)(q << 22);           // |v| <= 2^21
+  const float t = (float)v * (5.9604644775390625e-8f * 6.28318530717958648f);
+  const float t2 = t * t;
+  float sp = -1.0f / 39916800.0f;
+  sp = __builtin_fmaf(sp, t2, 1.0f / 362880.0f);
+  sp = __builtin_fmaf(sp, t2, -1.0f / 5040.0f);
+  sp = __builtin_fmaf(sp, t2, 1.0f / 120.0f);
+  sp = __builtin_fmaf(sp, t2, -1.0f / 6.0f);
+  const float sn = __builtin_fmaf(t * t2, sp, t);
+  float cp = -1.0f / 3628800.0f;
+  cp = __builtin_fmaf(cp, t2, 1.0f / 40320.0f);
+  cp = __builtin_fmaf(cp, t2, -1.0f / 720.0f);
+  cp = __builtin_fmaf(cp, t2, 1.0f / 24.0f);
+  cp = __builtin_fmaf(cp, t2, -0.5f);
+  const float cs = __builtin_fmaf(t2, cp, 1.0f);
+  const uint32_t quad = q & 3u;
+  float s, c;
+  if (quad == 0) {
+    s = sn;
+    c = cs;
+  } else if (quad == 1) {
+    s = cs;
+    c = -sn;
+  } else if (quad == 2) {
+    s = -sn;
+    c = -cs;
+  } else {
+    s = -cs;
+    c = sn;
+  }
+  *s_out = s;
+  *c_out = c;
+}
+
+// Three standard normals in fp32 from one Philox block (Box–Muller on
+// 24-bit uniforms): (w0, w1) -> a pair, (w2, w3) -> the cosine branch.
+// This is the displacement noise of a smart-MC move (the reference draws it
+// with curand_normal4 in fp32, device_function.cu:683).
+SCMF_HD void normal3f(const u32x4& w, float* n0, float* n1, float* n2) {
+  const float u0 = u01_24f(w.v[0]);  // (0, 1]
+  const float r0 = __builtin_sqrtf(-2.0f * logf_pos(u0));
+  float s0, c0;
+  sincosf_turn24(w.v[1] >> 8, &s0, &c0);
+  const float u2 = u01_24f(w.v[2]);
+  const float r1 = __builtin_sqrtf(-2.0f * logf_pos(u2));
+  float s1, c1;
+  sincosf_turn24(w.v[3] >> 8, &s1, &c1);
+  *n0 = r0 * c0;
+  *n1 = r0 * s0;
+  *n2 = r1 * c1;
+}
+
 // ---------------------------------------------------------------- exp
 // exp(x) in double.  k = nearest integer to x/ln2; r = x - k ln2 in two steps
 // (Cody & Waite 1980), |r| <= 0.347; exp(r) by its Taylor series to r^14

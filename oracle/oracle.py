@@ -57,6 +57,12 @@ def lib():
     L.orc_sincos_many.restype = None
     L.orc_normals.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int64, vp]
     L.orc_normals.restype = None
+    L.orc_logf_many.argtypes = [vp, vp, C.c_int64]
+    L.orc_logf_many.restype = None
+    L.orc_sincosf_turn24_many.argtypes = [vp, vp, vp, C.c_int64]
+    L.orc_sincosf_turn24_many.restype = None
+    L.orc_normals_f.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int64, vp]
+    L.orc_normals_f.restype = None
     L.orc_threads.restype = C.c_int
     _lib = L
     return L
@@ -150,6 +156,27 @@ def sincos_many(u):
 def normals(seed, system, chain, sweep, n):
     out = np.empty((n, 3), np.float64)
     lib().orc_normals(int(seed), int(system), int(chain), int(sweep), int(n), _ptr(out))
+    return out
+
+
+def logf_many(x):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().orc_logf_many(_ptr(x), _ptr(y), x.size)
+    return y
+
+
+def sincosf_turn24_many(k):
+    k = np.ascontiguousarray(k, np.uint32)
+    s = np.empty(k.size, np.float32)
+    c = np.empty(k.size, np.float32)
+    lib().orc_sincosf_turn24_many(_ptr(k), _ptr(s), _ptr(c), k.size)
+    return s, c
+
+
+def normals_f(seed, system, chain, sweep, n):
+    out = np.empty((n, 3), np.float32)
+    lib().orc_normals_f(int(seed), int(system), int(chain), int(sweep), int(n), _ptr(out))
     return out
 
 

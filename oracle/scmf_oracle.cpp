@@ -217,9 +217,8 @@ uint64_t chain_sweep(const Sys& s, const scmf_opts& o, float* pos, const float* 
       const float R = (float)s.R[iwtype];
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
       bond_forces(s, p, ibead, poly_index, pos, x, y, z, &fx, &fy, &fz);
-      double n0, n1, n2;
-      normal3(disp, &n0, &n1, &n2);
-      const float rx = (float)n0, ry = (float)n1, rz = (float)n2;
+      float rx, ry, rz;
+      normal3f(disp, &rx, &ry, &rz);
       dx = A * fx + rx * R;
       dy = A * fy + ry * R;
       dz = A * fz + rz * R;
@@ -461,6 +460,19 @@ void orc_normals(uint64_t seed, uint32_t system, uint32_t chain, uint32_t sweep,
   for (int64_t i = 0; i < n; ++i) {
     const u32x4 w = stream_block(seed, system, chain, sweep, (uint32_t)i, SLOT_DISP);
     normal3(w, &out[3 * i], &out[3 * i + 1], &out[3 * i + 2]);
+  }
+}
+void orc_logf_many(const float* x, float* y, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) y[i] = logf_pos(x[i]);
+}
+void orc_sincosf_turn24_many(const uint32_t* k, float* s, float* c, int64_t n) {
+  for (int64_t i = 0; i < n; ++i) sincosf_turn24(k[i], &s[i], &c[i]);
+}
+void orc_normals_f(uint64_t seed, uint32_t system, uint32_t chain, uint32_t sweep, int64_t n,
+                   float* out) {
+  for (int64_t i = 0; i < n; ++i) {
+    const u32x4 w = stream_block(seed, system, chain, sweep, (uint32_t)i, SLOT_DISP);
+    normal3f(w, &out[3 * i], &out[3 * i + 1], &out[3 * i + 2]);
   }
 }
 int orc_threads(void) {

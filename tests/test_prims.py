@@ -80,6 +80,38 @@ def test_sincos_accuracy():
     assert s[-6] == 0.0 and c[-6] == 1.0
 
 
+def test_logf_accuracy():
+    k = np.arange(1, 2**24 + 1, 97, dtype=np.uint64)
+    x = (k.astype(np.float64) * 2.0**-24).astype(np.float32)   # the uniforms fed to it
+    y = orc.logf_many(x).astype(np.float64)
+    want = np.log(x.astype(np.float64))
+    # absolute error against the exact log, in float ulps of the result
+    err = np.abs(y - want) / np.spacing(np.maximum(np.abs(want), 2.0**-24).astype(np.float32))
+    assert err.max() <= 2.0
+    assert orc.logf_many(np.array([1.0], np.float32))[0] == 0.0
+
+
+def test_sincosf_accuracy():
+    k = np.arange(0, 2**24, 61, dtype=np.uint32)
+    s, c = orc.sincosf_turn24_many(k)
+    th = 2 * np.pi * (k.astype(np.float64) * 2.0**-24)
+    assert np.abs(s - np.sin(th)).max() < 2.5e-7
+    assert np.abs(c - np.cos(th)).max() < 2.5e-7
+    s0, c0 = orc.sincosf_turn24_many(np.array([0, 1 << 22, 1 << 23, 3 << 22], np.uint32))
+    np.testing.assert_array_equal(s0, [0, 1, 0, -1])
+    np.testing.assert_array_equal(c0, [1, 0, -1, 0])
+
+
+def test_normals_f_moments():
+    n = orc.normals_f(seed=7, system=0, chain=3, sweep=11, n=400000).astype(np.float64).ravel()
+    assert abs(n.mean()) < 4 / np.sqrt(n.size)
+    assert abs(n.var() - 1) < 4 * np.sqrt(2 / n.size)
+    assert abs((n ** 4).mean() - 3) < 0.05
+    assert abs((np.abs(n) > 3).mean() - 0.0026998) < 0.0004
+    a = orc.normals_f(seed=9, system=2, chain=5, sweep=0, n=200000)
+    assert np.abs(np.corrcoef(a.T.astype(np.float64)) - np.eye(3)).max() < 0.01
+
+
 def test_normals_moments():
     n = orc.normals(seed=7, system=0, chain=3, sweep=11, n=400000).ravel()
     assert abs(n.mean()) < 4 * 1 / np.sqrt(n.size)
