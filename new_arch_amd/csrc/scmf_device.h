@@ -48,7 +48,49 @@ struct KParams {
   uint32_t system_id;
   int move_type;
   float trial_delta;
+  // device state of the system
+  float* x;
+  float* y;
+  float* z;
+  float* omega;
+  uint32_t* counts;              // uint16 pairs packed in words, or uint32
+  int64_t count_words;
+  int density_bits;
+  int omega_every_step;
+  unsigned long long* accepts;
+  uint32_t* overflow;
+  uint32_t* ctr;                 // [0] stream sweep word of the next sweep, [1] finished sweep blocks
 };
+
+// Which system and which of its blocks a workgroup works on.  Kernels that
+// serve a batch of systems get a table with one entry per workgroup; a single
+// system passes none.
+struct BlockRef {
+  uint32_t sys, local, nblk, pad;
+};
+
+__device__ __forceinline__ BlockRef block_ref(const BlockRef* bt) {
+  if (bt) return bt[blockIdx.x];
+  BlockRef r;
+  r.sys = 0;
+  r.local = blockIdx.x;
+  r.nblk = gridDim.x;
+  r.pad = 0;
+  return r;
+}
+
+// End of a sweep launch for one system: the last of its workgroups to finish
+// advances the system's sweep word (every workgroup has read it by then).
+__device__ __forceinline__ void finish_sweep_block(const KParams& kp, const BlockRef& br) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&kp.ctr[1], 1u) == br.nblk - 1) {
+      atomicExch(&kp.ctr[1], 0u);
+      atomicAdd(&kp.ctr[0], 1u);
+    }
+  }
+}
 
 __device__ __forceinline__ uint32_t bin_cell(const KParams& kp, double rx, double ry, double rz,
                                              uint32_t type) {

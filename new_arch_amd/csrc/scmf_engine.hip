@@ -38,10 +38,14 @@ constexpr int BLOCK = 256;
 constexpr int MAX_INIT_N = 4096;  // occupancy bitmask of the initial walk
 
 // initialize_coord, device_function.cu:140-207, one lane per chain.
-__global__ void __launch_bounds__(BLOCK) k_init_coords(KParams kp, float* __restrict__ x,
-                                                       float* __restrict__ y,
-                                                       float* __restrict__ z) {
-  const int64_t chain = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+__global__ void __launch_bounds__(BLOCK) k_init_coords(const KParams* __restrict__ Ps,
+                                                       const BlockRef* __restrict__ bt) {
+  const BlockRef br = block_ref(bt);
+  const KParams& kp = Ps[br.sys];
+  float* __restrict__ x = kp.x;
+  float* __restrict__ y = kp.y;
+  float* __restrict__ z = kp.z;
+  const int64_t chain = (int64_t)br.local * BLOCK + threadIdx.x;
   if (chain >= kp.n_chains) return;
   const int p = arch_of_chain(kp, chain);
   const int N = kp.len[p];
@@ -217,13 +221,16 @@ __device__ __forceinline__ void wave_count(uint32_t acc, unsigned long long* acc
 
 // mc_polymer_move_arr, one lane per chain, positions read and written in
 // global memory (used when a chain tile does not fit in LDS).
-__global__ void __launch_bounds__(BLOCK) k_sweep_global(KParams kp, float* __restrict__ x,
-                                                        float* __restrict__ y,
-                                                        float* __restrict__ z,
-                                                        const float* __restrict__ omega,
-                                                        uint32_t sweep,
-                                                        unsigned long long* __restrict__ accepts) {
-  const int64_t chain = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+__global__ void __launch_bounds__(BLOCK) k_sweep_global(const KParams* __restrict__ Ps,
+                                                        const BlockRef* __restrict__ bt) {
+  const BlockRef br = block_ref(bt);
+  const KParams& kp = Ps[br.sys];
+  float* __restrict__ x = kp.x;
+  float* __restrict__ y = kp.y;
+  float* __restrict__ z = kp.z;
+  const float* __restrict__ omega = kp.omega;
+  const uint32_t sweep = kp.ctr[0];
+  const int64_t chain = (int64_t)br.local * BLOCK + threadIdx.x;
   uint32_t acc = 0;
   if (chain < kp.n_chains) {
     const int p = arch_of_chain(kp, chain);
@@ -233,21 +240,26 @@ __global__ void __launch_bounds__(BLOCK) k_sweep_global(KParams kp, float* __res
     const uint32_t id = kp.slot_chain[chain];
     for (int i = 0; i < length; i++) acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, omega);
   }
-  wave_count(acc, accepts);
+  wave_count(acc, kp.accepts);
+  finish_sweep_block(kp, br);
 }
 
 // mc_polymer_move_arr with the wavefront's 64 chains staged in LDS: one lane
 // per chain, bead b of lane l at LDS word b*64 + l (conflict-free for any
 // per-lane bead index).  Each lane touches only its own column, so no barrier
 // is needed; global memory sees one read and one write of every coordinate.
-__global__ void __launch_bounds__(64) k_sweep_lds(KParams kp, float* __restrict__ x,
-                                                  float* __restrict__ y, float* __restrict__ z,
-                                                  const float* __restrict__ omega,
-                                                  uint32_t sweep, int nmax,
-                                                  unsigned long long* __restrict__ accepts) {
+__global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps,
+                                                  const BlockRef* __restrict__ bt, int nmax) {
   extern __shared__ float lds[];
+  const BlockRef br = block_ref(bt);
+  const KParams& kp = Ps[br.sys];
+  float* __restrict__ x = kp.x;
+  float* __restrict__ y = kp.y;
+  float* __restrict__ z = kp.z;
+  const float* __restrict__ omega = kp.omega;
+  const uint32_t sweep = kp.ctr[0];
   const int lane = threadIdx.x;
-  const int64_t chain = (int64_t)blockIdx.x * 64 + lane;
+  const int64_t chain = (int64_t)br.local * 64 + lane;
   uint32_t acc = 0;
   if (chain < kp.n_chains) {
     const int p = arch_of_chain(kp, chain);
@@ -270,7 +282,8 @@ __global__ void __launch_bounds__(64) k_sweep_lds(KParams kp, float* __restrict_
       z[off + b] = sz[b * 64];
     }
   }
-  wave_count(acc, accepts);
+  wave_count(acc, kp.accepts);
+  finish_sweep_block(kp, br);
 }
 
 // mc_polymer_move_arr with L lanes per chain (G = 64/L chains per wavefront).
@@ -290,14 +303,17 @@ __global__ void __launch_bounds__(64) k_sweep_lds(KParams kp, float* __restrict_
 // W[N+1] (level per bead during the scan, then per-level fill counts);
 // start[N+2] (first list entry of each level); list[N] (moves by level).
 template <int L, typename I>
-__global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restrict__ x,
-                                                    float* __restrict__ y,
-                                                    float* __restrict__ z,
-                                                    const float* __restrict__ omega,
-                                                    uint32_t sweep, int nmax,
-                                                    unsigned long long* __restrict__ accepts) {
+__global__ void __launch_bounds__(64) k_sweep_sched(const KParams* __restrict__ Ps,
+                                                    const BlockRef* __restrict__ bt, int nmax) {
   constexpr int G = 64 / L;
   extern __shared__ float lds[];
+  const BlockRef br = block_ref(bt);
+  const KParams& kp = Ps[br.sys];
+  float* __restrict__ x = kp.x;
+  float* __restrict__ y = kp.y;
+  float* __restrict__ z = kp.z;
+  const float* __restrict__ omega = kp.omega;
+  const uint32_t sweep = kp.ctr[0];
   const int lane = threadIdx.x;
   const int slot = lane / L, r = lane % L;
   float* sx = lds;
@@ -309,7 +325,7 @@ __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restric
   I* W = lev + nmax;
   I* start = W + nmax + 1;
   I* list = start + nmax + 2;
-  const int64_t chain = (int64_t)blockIdx.x * G + slot;
+  const int64_t chain = (int64_t)br.local * G + slot;
   const bool active = chain < kp.n_chains;
   int p = 0, length = 0;
   int64_t off = 0;
@@ -382,7 +398,8 @@ __global__ void __launch_bounds__(64) k_sweep_sched(KParams kp, float* __restric
     y[off + b] = sy[b * G + slot];
     z[off + b] = sz[b * G + slot];
   }
-  wave_count(acc, accepts);
+  wave_count(acc, kp.accepts);
+  finish_sweep_block(kp, br);
 }
 
 template <int L, typename I>
@@ -391,28 +408,41 @@ size_t sched_lds_bytes(int nmax) {
   return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (4 * nmax + 3) * sizeof(I);
 }
 
+// Clear each system's counts (a kernel, not a memset, so it can sit in a graph).
+__global__ void __launch_bounds__(BLOCK) k_zero_counts(const KParams* __restrict__ Ps,
+                                                       const BlockRef* __restrict__ bt) {
+  const BlockRef br = block_ref(bt);
+  const KParams& kp = Ps[br.sys];
+  const int64_t stride = (int64_t)br.nblk * BLOCK;
+  for (int64_t i = (int64_t)br.local * BLOCK + threadIdx.x; i < kp.count_words; i += stride)
+    kp.counts[i] = 0u;
+}
+
 __global__ void __launch_bounds__(BLOCK) k_zero(uint32_t* __restrict__ p, int64_t n) {
   const int64_t stride = (int64_t)gridDim.x * BLOCK;
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) p[i] = 0u;
 }
 
 // coord_to_density, device_function.cu:479-508, one lane per bead.
-template <int BITS>
-__global__ void __launch_bounds__(BLOCK) k_grid(KParams kp, const float* __restrict__ x,
-                                                const float* __restrict__ y,
-                                                const float* __restrict__ z,
-                                                uint32_t* __restrict__ words,
-                                                uint32_t* __restrict__ overflow) {
-  const int64_t stride = (int64_t)gridDim.x * BLOCK;
-  for (int64_t g = (int64_t)blockIdx.x * BLOCK + threadIdx.x; g < kp.n_beads; g += stride) {
+__global__ void __launch_bounds__(BLOCK) k_grid(const KParams* __restrict__ Ps,
+                                                const BlockRef* __restrict__ bt) {
+  const BlockRef br = block_ref(bt);
+  const KParams& kp = Ps[br.sys];
+  const float* __restrict__ x = kp.x;
+  const float* __restrict__ y = kp.y;
+  const float* __restrict__ z = kp.z;
+  uint32_t* __restrict__ words = kp.counts;
+  const bool b32 = kp.density_bits == 32;
+  const int64_t stride = (int64_t)br.nblk * BLOCK;
+  for (int64_t g = (int64_t)br.local * BLOCK + threadIdx.x; g < kp.n_beads; g += stride) {
     const uint32_t t = kp.btype[g];
     const uint32_t cell = bin_cell(kp, (double)x[g], (double)y[g], (double)z[g], t);
-    if (BITS == 32) {
+    if (b32) {
       atomicAdd(&words[cell], 1u);
     } else {
       const uint32_t sh = (cell & 1u) * 16u;
       const uint32_t old = atomicAdd(&words[cell >> 1], 1u << sh);
-      if (((old >> sh) & 0xFFFFu) == 0xFFFFu) atomicOr(overflow, 1u);
+      if (((old >> sh) & 0xFFFFu) == 0xFFFFu) atomicOr(kp.overflow, 1u);
     }
   }
 }
@@ -425,15 +455,24 @@ __device__ __forceinline__ uint32_t load_count(const uint32_t* words, uint32_t i
 
 // omega_field_update, device_function.cu:373-412: fp64 in the reference's
 // operation order, rounded once to fp32.
-template <int BITS>
-__global__ void __launch_bounds__(BLOCK) k_omega(KParams kp, const uint32_t* __restrict__ words,
-                                                 float* __restrict__ omega) {
+// With `cadence` set, a system that keeps the reference cadence skips the
+// update after sweeps whose count is a multiple of 10 (test.cpp:42-43).
+__global__ void __launch_bounds__(BLOCK) k_omega(const KParams* __restrict__ Ps,
+                                                 const BlockRef* __restrict__ bt, int cadence) {
+  const BlockRef br = block_ref(bt);
+  const KParams& kp = Ps[br.sys];
+  if (cadence && !kp.omega_every_step && kp.ctr[0] % 10u == 0u) return;
+  const uint32_t* __restrict__ words = kp.counts;
+  float* __restrict__ omega = kp.omega;
   const uint32_t C = kp.n_cells;
   const int T = kp.T;
-  const int64_t stride = (int64_t)gridDim.x * BLOCK;
-  for (int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < C; c += stride) {
+  const bool b32 = kp.density_bits == 32;
+  const int64_t stride = (int64_t)br.nblk * BLOCK;
+  for (int64_t c = (int64_t)br.local * BLOCK + threadIdx.x; c < C; c += stride) {
     double n[MAX_T];
-    for (int s = 0; s < T; ++s) n[s] = (double)load_count<BITS>(words, (uint32_t)c + s * C);
+    for (int s = 0; s < T; ++s)
+      n[s] = (double)(b32 ? load_count<32>(words, (uint32_t)c + s * C)
+                          : load_count<16>(words, (uint32_t)c + s * C));
     for (int t = 0; t < T; ++t) {
       double w = 0;
       for (int s = 0; s < T; ++s) w += n[s] * kp.fst[t];
@@ -634,6 +673,8 @@ struct scmf_sys {
   bool sorted = false;
   int sort_every = 0;
   uint64_t n_sorts = 0;
+  KParams* d_kp = nullptr;  // device copy of kp read by the batch-capable kernels
+  uint32_t* d_ctr = nullptr;
   size_t lds_bytes = 0; // LDS of the sweep kernel
   const SweepVariant* variant = nullptr;
   float* x = nullptr;
@@ -808,6 +849,21 @@ int ensure_staging(scmf_sys* s, int64_t beads) {
 
 constexpr int64_t STAGE_BEADS = (int64_t)1 << 24;
 
+// Refresh the device copy of the system parameters (after creation, a
+// reordering, or a re-keyed stream).
+int sync_kp(scmf_sys* s) {
+  HIP_TRY(hipMemcpyAsync(s->d_kp, &s->kp, sizeof(KParams), hipMemcpyHostToDevice, s->stream));
+  return SCMF_OK;
+}
+
+// Set the device sweep word (the stream position of the next sweep).
+int set_ctr(scmf_sys* s, uint32_t next_sweep) {
+  const uint32_t v[2] = {next_sweep, 0u};
+  HIP_TRY(hipMemcpyAsync(s->d_ctr, v, sizeof v, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return SCMF_OK;
+}
+
 size_t lds1_bytes(int nmax) { return (size_t)3 * nmax * 64 * sizeof(float); }
 size_t none_bytes(int) { return 0; }
 
@@ -873,22 +929,20 @@ int choose_sweep(scmf_sys* s) {
 
 int launch_sweep(scmf_sys* s) {
   const SweepVariant* v = s->variant;
-  const uint32_t sweep = (uint32_t)s->sweeps;
+  const BlockRef* none = nullptr;
   if (v->L == 0) {
     hipLaunchKernelGGL(k_sweep_global, dim3(grid_for(s->n_chains)), dim3(BLOCK), 0, s->stream,
-                       s->kp, s->x, s->y, s->z, s->omega, sweep, s->d_accepts);
+                       (const KParams*)s->d_kp, none);
   } else {
     const int G = 64 / v->L;
     const dim3 grid((unsigned)((s->n_chains + G - 1) / G));
-    void* args[] = {(void*)&s->kp, (void*)&s->x, (void*)&s->y, (void*)&s->z,
-                    (void*)&s->omega, (void*)&sweep, (void*)&s->nmax, (void*)&s->d_accepts};
+    const KParams* kpp = s->d_kp;
+    void* args[] = {(void*)&kpp, (void*)&none, (void*)&s->nmax};
     HIP_TRY(hipLaunchKernel(v->fn, grid, dim3(64), args, s->lds_bytes, s->stream));
   }
   HIP_TRY(hipGetLastError());
   return SCMF_OK;
 }
-
-
 
 // Tile geometry of the tiled gridding.  The halo covers the reach of a chain
 // from its middle bead (0.75 Re sqrt((N-1)/(Nref-1)), in cells).  Per axis the
@@ -999,6 +1053,10 @@ int sort_chains(scmf_sys* s) {
   std::swap(s->z, s->z2);
   std::swap(s->slot_chain, s->slot_chain2);
   s->kp.slot_chain = s->slot_chain;
+  s->kp.x = s->x;
+  s->kp.y = s->y;
+  s->kp.z = s->z;
+  if ((rc = sync_kp(s))) return rc;
   s->sorted = true;
   s->n_sorts += 1;
   return SCMF_OK;
@@ -1011,7 +1069,8 @@ int free_all(scmf_sys* s) {
                   (void*)s->staging, (void*)s->slot_chain, (void*)s->chain_slot,
                   (void*)s->slot_chain2, (void*)s->btype, (void*)s->keys, (void*)s->tcount,
                   (void*)s->tstart, (void*)s->tfill, (void*)s->new_slot, (void*)s->x2,
-                  (void*)s->y2, (void*)s->z2, (void*)s->scratch, (void*)s->d_fallback})
+                  (void*)s->y2, (void*)s->z2, (void*)s->scratch, (void*)s->d_fallback,
+                  (void*)s->d_kp, (void*)s->d_ctr})
     if (p) (void)hipFree(p);
   for (Timed& t : s->pending) {
     (void)hipEventDestroy(t.a);
@@ -1087,9 +1146,26 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
       (rc = alloc((void**)&s->btype, (size_t)B)) ||
       (rc = alloc((void**)&s->d_fallback, sizeof(unsigned long long))))
     return bail(rc);
+  if ((rc = alloc((void**)&s->d_kp, sizeof(KParams))) ||
+      (rc = alloc((void**)&s->d_ctr, 2 * sizeof(uint32_t))))
+    return bail(rc);
   s->kp.slot_chain = s->slot_chain;
   s->kp.chain_slot = s->chain_slot;
   s->kp.btype = s->btype;
+  s->kp.x = s->x;
+  s->kp.y = s->y;
+  s->kp.z = s->z;
+  s->kp.omega = s->omega;
+  s->kp.counts = s->counts;
+  s->kp.count_words = s->count_words;
+  s->kp.density_bits = o.density_bits;
+  s->kp.omega_every_step = o.omega_every_step ? 1 : 0;
+  s->kp.accepts = s->d_accepts;
+  s->kp.overflow = s->d_overflow;
+  s->kp.ctr = s->d_ctr;
+  if (hipMemsetAsync(s->d_ctr, 0, 2 * sizeof(uint32_t), s->stream) != hipSuccess ||
+      sync_kp(s) != SCMF_OK)
+    return bail(fail(SCMF_E_HIP, "parameter upload failed"));
   hipLaunchKernelGGL(k_iota, dim3(grid_strided(s->n_chains)), dim3(BLOCK), 0, s->stream,
                      s->slot_chain, s->n_chains);
   hipLaunchKernelGGL(k_iota, dim3(grid_strided(s->n_chains)), dim3(BLOCK), 0, s->stream,
@@ -1104,8 +1180,8 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
       hipMemsetAsync(s->counts, 0, sizeof(uint32_t) * s->count_words, s->stream) != hipSuccess)
     return bail(fail(SCMF_E_HIP, "initial clear failed"));
   if (o.init_coords) {
-    hipLaunchKernelGGL(k_init_coords, dim3(grid_for(s->n_chains)), dim3(BLOCK), 0, s->stream, s->kp,
-                       s->x, s->y, s->z);
+    hipLaunchKernelGGL(k_init_coords, dim3(grid_for(s->n_chains)), dim3(BLOCK), 0, s->stream,
+                       (const KParams*)s->d_kp, (const BlockRef*)nullptr);
     if (hipGetLastError() != hipSuccess) return bail(fail(SCMF_E_HIP, "init launch failed"));
   } else {
     if (hipMemsetAsync(s->x, 0, sizeof(float) * B, s->stream) != hipSuccess ||
@@ -1203,8 +1279,8 @@ int scmf_grid_density(scmf_sys* s) {
     int rc = s->ev_begin(K_GRID, &t);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_zero, dim3(grid_strided(s->count_words)), dim3(BLOCK), 0, s->stream,
-                     s->counts, s->count_words);
+  hipLaunchKernelGGL(k_zero_counts, dim3(grid_strided(s->count_words)), dim3(BLOCK), 0, s->stream,
+                     (const KParams*)s->d_kp, (const BlockRef*)nullptr);
   const bool b32 = s->opts.density_bits == 32;
   if (s->tile_lds && s->sorted) {
     const dim3 tiles((unsigned)s->geo.ntiles);
@@ -1224,12 +1300,9 @@ int scmf_grid_density(scmf_sys* s) {
       hipLaunchKernelGGL(k_tiles_sum<16>, dim3(grid_strided(s->count_words)), dim3(256), 0,
                          s->stream, s->kp, s->geo, s->scratch, s->counts, s->d_overflow,
                          s->count_words);
-  } else if (b32) {
-    hipLaunchKernelGGL(k_grid<32>, dim3(grid_strided(s->n_beads)), dim3(BLOCK), 0, s->stream,
-                       s->kp, s->x, s->y, s->z, s->counts, s->d_overflow);
   } else {
-    hipLaunchKernelGGL(k_grid<16>, dim3(grid_strided(s->n_beads)), dim3(BLOCK), 0, s->stream,
-                       s->kp, s->x, s->y, s->z, s->counts, s->d_overflow);
+    hipLaunchKernelGGL(k_grid, dim3(grid_strided(s->n_beads)), dim3(BLOCK), 0, s->stream,
+                       (const KParams*)s->d_kp, (const BlockRef*)nullptr);
   }
   HIP_TRY(hipGetLastError());
   if (s->timing) return s->ev_end(&t);
@@ -1248,12 +1321,8 @@ int scmf_update_omega(scmf_sys* s) {
     int rc = s->ev_begin(K_OMEGA, &t);
     if (rc) return rc;
   }
-  if (s->opts.density_bits == 32)
-    hipLaunchKernelGGL(k_omega<32>, dim3(grid_strided(s->n_cells)), dim3(BLOCK), 0, s->stream,
-                       s->kp, s->counts, s->omega);
-  else
-    hipLaunchKernelGGL(k_omega<16>, dim3(grid_strided(s->n_cells)), dim3(BLOCK), 0, s->stream,
-                       s->kp, s->counts, s->omega);
+  hipLaunchKernelGGL(k_omega, dim3(grid_strided(s->n_cells)), dim3(BLOCK), 0, s->stream,
+                     (const KParams*)s->d_kp, (const BlockRef*)nullptr, 0);
   HIP_TRY(hipGetLastError());
   if (s->timing) return s->ev_end(&t);
   return SCMF_OK;
@@ -1368,7 +1437,9 @@ int scmf_set_rng(scmf_sys* s, uint64_t seed, uint64_t next_sweep) {
   s->opts.seed = seed;
   s->kp.seed = seed;
   s->sweeps = next_sweep;
-  return SCMF_OK;
+  int rc = sync_kp(s);
+  if (rc) return rc;
+  return set_ctr(s, (uint32_t)next_sweep);
 }
 
 int scmf_stats(scmf_sys* s, scmf_stats_t* out) {
@@ -1481,6 +1552,7 @@ int scmf_autotune(scmf_sys* s, int32_t sweeps_per_variant, int32_t* chosen, floa
     HIP_TRY(hipMemcpyAsync(s->x, backup, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->y, backup + B, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
     HIP_TRY(hipMemcpyAsync(s->z, backup + 2 * B, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
+    if ((rc = set_ctr(s, (uint32_t)sweeps0))) break;
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
