@@ -66,6 +66,75 @@ def cpu_baseline(spec, opts, target_s=15.0):
                       f"x {sweeps} sweeps, {dt:.1f} s; gridding and omega not included"}
 
 
+def batched_vs_plain(steps, warmup, n_systems=64):
+    """The batched form against the same job mix run one system at a time with
+    the plain per-system kernels: aggregate bead-moves/s of each."""
+    from new_arch_amd import scmf
+    from new_arch_amd import workloads as W
+    from new_arch_amd.abi import make_opts
+
+    mix = W.batched_mix(n_systems=n_systems, seed=0)
+
+    def members():
+        out = []
+        for spec, seed, sid in mix:
+            g = scmf.System(spec, make_opts(seed=seed, system_id=sid, omega_every_step=True,
+                                            density_bits=16))
+            g.grid_density()
+            g.update_omega()
+            out.append(g)
+        for g in out:
+            g.sync()
+        return out
+
+    beads = sum(spec.n_beads for spec, _, _ in mix)
+    plain = members()
+    for g in plain:                      # one system at a time, plain kernels
+        for k in range(warmup):
+            g.step(k + 1)
+        g.sync()
+    t0 = time.perf_counter()
+    for g in plain:
+        for k in range(steps):
+            g.step(warmup + k + 1)
+        g.sync()
+    t_plain = time.perf_counter() - t0
+    for g in plain:
+        g.close()
+    batch_members = members()
+    b = scmf.Batch(batch_members)
+    best = None
+    for lanes in (1, 2, 4, 8):           # autotune the shared chain-to-wavefront shape
+        b.set_lanes(lanes)
+        b.step(1)
+        b.sync()
+        t = time.perf_counter()
+        b.step(3)
+        b.sync()
+        dt = time.perf_counter() - t
+        if best is None or dt < best[1]:
+            best = (lanes, dt)
+    b.set_lanes(best[0])
+    b.step(max(0, warmup - 4))
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    t_batch = time.perf_counter() - t0
+    b.close()
+    for g in batch_members:
+        g.close()
+    plain_v = beads * steps / t_plain
+    batch_v = beads * steps / t_batch
+    log(f"batched mix: {n_systems} systems, {beads} beads; plain {plain_v:.3e}, "
+        f"batched {batch_v:.3e} bead-moves/s ({batch_v / plain_v:.1f}x), lanes {best[0]}")
+    return {"workload": "batched_mix", "systems": n_systems, "beads": beads, "steps": steps,
+            "plain_bead_moves_per_s": plain_v, "batched_bead_moves_per_s": batch_v,
+            "batched_over_plain": batch_v / plain_v, "lanes_per_chain": best[0],
+            "plain_ms_per_step_all_systems": t_plain * 1e3 / steps,
+            "batched_ms_per_step": t_batch * 1e3 / steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -73,6 +142,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="large_diblock_melt")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs one system per GPU and measures a single GPU; "
@@ -155,6 +225,9 @@ def main():
         "acceptance": st.accepts / max(st.moves, 1),
     }
     g.close()
+    if not args.no_batched:
+        log("batched vs plain")
+        out["batched"] = batched_vs_plain(steps=50, warmup=5)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(spec, opts)

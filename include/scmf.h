@@ -197,6 +197,20 @@ int scmf_write_density_host(const char* path, const int32_t n[3], const double L
 /* vtk.c:4-81: density_*.dat -> STRUCTURED_POINTS VTK of phi_0 - phi_1. */
 int scmf_density_to_vtk(const char* density_path, const char* vtk_path);
 
+/* ---- batches: many independent systems advanced together -------------------
+ * No counterpart in the reference (one system per process).  Each step runs
+ * one launch per phase across all members (sweep, clear, gridding, omega with
+ * every member's own cadence) from a captured HIP graph.  Every member ends
+ * up exactly where the same number of scmf_step calls on it alone would
+ * leave it.  Call scmf_batch_sync before using a member's handle again. */
+typedef struct scmf_batch scmf_batch;
+int scmf_batch_create(scmf_batch** out, scmf_sys* const* members, int32_t n_members);
+int scmf_batch_step(scmf_batch* batch, int32_t n_steps);
+int scmf_batch_sync(scmf_batch* batch);
+/* Force one sweep-kernel shape for every member (0 = each member's own). */
+int scmf_batch_set_lanes(scmf_batch* batch, int32_t lanes_per_chain);
+void scmf_batch_destroy(scmf_batch* batch);
+
 const char* scmf_last_error(void);
 const char* scmf_version(void);
 

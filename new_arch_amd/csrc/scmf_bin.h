@@ -58,7 +58,7 @@ SCMF_HD int fold_exact(double r, double L, double d, int n) {
 }
 
 // Same result as fold_exact for every input.
-SCMF_HD int fold_fast(double r, const Axis& a) {
+SCMF_HD int fold_fast(double r, const Axis a) {
   const double q1 = r * a.invL;
   const double m = __builtin_floor(q1);
   const double f1 = q1 - m;  // exact (Sterbenz) for |q1| < 2^52

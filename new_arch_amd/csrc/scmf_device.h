@@ -14,7 +14,20 @@
 
 #pragma clang fp contract(off)
 
+// Global-address-space pointers: loads through them are vector-memory
+// instructions only (a pointer read from memory would otherwise be flat and
+// also wait on the LDS counter).
+#define SCMF_GLOBAL __attribute__((address_space(1)))
+// Constant address space: the kernels read the system parameters through it,
+// so the loads are invariant (hoisted across barriers) and scalar when uniform.
+#define SCMF_CONST __attribute__((address_space(4)))
+
 namespace scmf {
+
+template <class T>
+__device__ __forceinline__ SCMF_GLOBAL T* gptr(T* p) {
+  return (SCMF_GLOBAL T*)p;
+}
 
 constexpr int MAX_T = 8;   // monomer types
 constexpr int MAX_P = 16;  // polymer architectures
@@ -57,10 +70,14 @@ struct KParams {
   int64_t count_words;
   int density_bits;
   int omega_every_step;
-  unsigned long long* accepts;
+  unsigned long long* accepts;    // ACC_SLOTS partial counts (summed on the host)
   uint32_t* overflow;
-  uint32_t* ctr;                 // [0] stream sweep word of the next sweep, [1] finished sweep blocks
+  uint32_t* ctr;                 // [0] stream sweep word of the next sweep
 };
+
+// Accepted moves are counted in this many separate words, chosen by
+// workgroup, so the workgroups of a sweep do not queue on one address.
+constexpr int ACC_SLOTS = 256;
 
 // Which system and which of its blocks a workgroup works on.  Kernels that
 // serve a batch of systems get a table with one entry per workgroup; a single
@@ -68,6 +85,13 @@ struct KParams {
 struct BlockRef {
   uint32_t sys, local, nblk, pad;
 };
+
+using KPC = const SCMF_CONST KParams;
+
+// The parameters of system `sys` seen through the constant address space.
+__device__ __forceinline__ KPC& param(const KParams* Ps, uint32_t sys) {
+  return *(KPC*)(Ps + sys);
+}
 
 __device__ __forceinline__ BlockRef block_ref(const BlockRef* bt) {
   if (bt) return bt[blockIdx.x];
@@ -79,35 +103,41 @@ __device__ __forceinline__ BlockRef block_ref(const BlockRef* bt) {
   return r;
 }
 
-// End of a sweep launch for one system: the last of its workgroups to finish
-// advances the system's sweep word (every workgroup has read it by then).
-__device__ __forceinline__ void finish_sweep_block(const KParams& kp, const BlockRef& br) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    if (atomicAdd(&kp.ctr[1], 1u) == br.nblk - 1) {
-      atomicExch(&kp.ctr[1], 0u);
-      atomicAdd(&kp.ctr[0], 1u);
-    }
-  }
+
+
+// Binning constants of one axis, copied out of the parameters (whatever
+// address space they are read from).
+template <class KP>
+__device__ __forceinline__ Axis axis(const KP& kp, int a) {
+  Axis r;
+  r.L = kp.ax[a].L;
+  r.d = kp.ax[a].d;
+  r.invL = kp.ax[a].invL;
+  r.invd = kp.ax[a].invd;
+  r.guard2 = kp.ax[a].guard2;
+  r.n = kp.ax[a].n;
+  return r;
 }
 
-__device__ __forceinline__ uint32_t bin_cell(const KParams& kp, double rx, double ry, double rz,
+template <class KP>
+__device__ __forceinline__ uint32_t bin_cell(const KP& kp, double rx, double ry, double rz,
                                              uint32_t type) {
-  const int xi = fold_fast(rx, kp.ax[0]);
-  const int yi = fold_fast(ry, kp.ax[1]);
-  const int zi = fold_fast(rz, kp.ax[2]);
+  const int xi = fold_fast(rx, axis(kp, 0));
+  const int yi = fold_fast(ry, axis(kp, 1));
+  const int zi = fold_fast(rz, axis(kp, 2));
   return type * kp.n_cells + (uint32_t)(xi + yi * kp.nx + zi * kp.nx * kp.ny);
 }
 
 // polymer_number_to_index (device_function.cu:72-93): architecture of a chain.
-__device__ __forceinline__ int arch_of_chain(const KParams& kp, int64_t chain) {
+template <class KP>
+__device__ __forceinline__ int arch_of_chain(const KP& kp, int64_t chain) {
   int p = 0;
   while (p < kp.P - 1 && chain >= kp.chain_base[p + 1]) ++p;
   return p;
 }
 
-__device__ __forceinline__ int arch_of_bead(const KParams& kp, int64_t bead) {
+template <class KP>
+__device__ __forceinline__ int arch_of_bead(const KP& kp, int64_t bead) {
   int p = 0;
   while (p < kp.P - 1 && bead >= kp.bead_base[p + 1]) ++p;
   return p;

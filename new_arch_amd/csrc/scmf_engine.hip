@@ -41,10 +41,10 @@ constexpr int MAX_INIT_N = 4096;  // occupancy bitmask of the initial walk
 __global__ void __launch_bounds__(BLOCK) k_init_coords(const KParams* __restrict__ Ps,
                                                        const BlockRef* __restrict__ bt) {
   const BlockRef br = block_ref(bt);
-  const KParams& kp = Ps[br.sys];
-  float* __restrict__ x = kp.x;
-  float* __restrict__ y = kp.y;
-  float* __restrict__ z = kp.z;
+  KPC& kp = param(Ps, br.sys);
+  SCMF_GLOBAL float* __restrict__ x = gptr((float*)kp.x);
+  SCMF_GLOBAL float* __restrict__ y = gptr((float*)kp.y);
+  SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
   const int64_t chain = (int64_t)br.local * BLOCK + threadIdx.x;
   if (chain >= kp.n_chains) return;
   const int p = arch_of_chain(kp, chain);
@@ -78,9 +78,9 @@ __global__ void __launch_bounds__(BLOCK) k_init_coords(const KParams* __restrict
 
 // Positions of one chain, seen by one lane.
 struct GlobalChain {  // planar global arrays, chain starting at `base`
-  float* x;
-  float* y;
-  float* z;
+  SCMF_GLOBAL float* x;
+  SCMF_GLOBAL float* y;
+  SCMF_GLOBAL float* z;
   __device__ float X(int b) const { return x[b]; }
   __device__ float Y(int b) const { return y[b]; }
   __device__ float Z(int b) const { return z[b]; }
@@ -107,21 +107,23 @@ struct LdsChain {  // LDS columns: bead b of this lane's chain at [b * STRIDE]
 
 // Calls f(j) for every neighbour j of bead b, ascending (the order of
 // init_config.cu:330-336): computed for linear chains, looked up otherwise.
-template <class F>
-__device__ __forceinline__ void for_neighbours(const KParams& kp, int p, int b, int len, F&& f) {
+template <class KP, class F>
+__device__ __forceinline__ void for_neighbours(const KP& kp, int p, int b, int len, F&& f) {
   if (kp.linear[p]) {
     if (b > 0) f(b - 1);
     if (b < len - 1) f(b + 1);
   } else {
-    const int k0 = kp.nbr_start[kp.mono_off[p] + b], k1 = kp.nbr_start[kp.mono_off[p] + b + 1];
-    for (int k = k0; k < k1; ++k) f(kp.nbr[k]);
+    const SCMF_GLOBAL int* ns = gptr((const int*)kp.nbr_start);
+    const SCMF_GLOBAL int* nb = gptr((const int*)kp.nbr);
+    const int k0 = ns[kp.mono_off[p] + b], k1 = ns[kp.mono_off[p] + b + 1];
+    for (int k = k0; k < k1; ++k) f(nb[k]);
   }
 }
 
 // add_bond_forces_arr, device_function.cu:589-634: neighbour difference in
 // fp32, scaled and accumulated in fp64, added to the fp32 force.
-template <class V>
-__device__ __forceinline__ void bond_force(const KParams& kp, const V& c, int p, int b, int len,
+template <class KP, class V>
+__device__ __forceinline__ void bond_force(const KP& kp, const V& c, int p, int b, int len,
                                            float px, float py, float pz, float* fx, float* fy,
                                            float* fz) {
   const double harm = kp.harm;
@@ -142,14 +144,14 @@ constexpr int MAX_DEG = 8;  // bonds per bead handled by the sweep kernels
 // trial_move_smc_arr (:637-717), calc_delta_energy_arr (:562-588),
 // calc_delta_bonded_energy_arr (:509-561) and som_accept (:444-454) inlined.
 // Returns 1 if accepted.
-template <class V>
-__device__ __forceinline__ uint32_t one_move(const KParams& kp, const V& c, int p, int length,
+template <class KP, class V>
+__device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, int p, int length,
                                              uint32_t chain, uint32_t sweep, uint32_t i,
-                                             const float* __restrict__ omega) {
+                                             const SCMF_GLOBAL float* __restrict__ omega) {
   const u32x4 pick = stream_block(kp.seed, kp.system_id, chain, sweep, i, SLOT_PICK);
   int ibead = (int)(u01_53(pick.v[0], pick.v[1]) * length);
   if (ibead == length) ibead = length - 1;
-  const uint32_t iwtype = (uint32_t)kp.monotype[kp.mono_off[p] + ibead];
+  const uint32_t iwtype = (uint32_t)gptr((const int*)kp.monotype)[kp.mono_off[p] + ibead];
   const u32x4 disp = stream_block(kp.seed, kp.system_id, chain, sweep, i, SLOT_DISP);
   const float ox = c.X(ibead), oy = c.Y(ibead), oz = c.Z(ibead);
   float dx, dy, dz;
@@ -216,7 +218,18 @@ __device__ __forceinline__ uint32_t one_move(const KParams& kp, const V& c, int 
 
 __device__ __forceinline__ void wave_count(uint32_t acc, unsigned long long* accepts) {
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  if ((threadIdx.x & 63) == 0 && acc) atomicAdd(accepts, (unsigned long long)acc);
+  if ((threadIdx.x & 63) == 0 && acc)
+    atomicAdd(&accepts[(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (ACC_SLOTS - 1)],
+              (unsigned long long)acc);
+}
+
+// Advance the stream sweep word of systems [0, n) after their sweep.
+__global__ void __launch_bounds__(64) k_advance(const KParams* __restrict__ Ps, int n) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i < n) {
+    uint32_t* c = Ps[i].ctr;
+    c[0] = c[0] + 1u;
+  }
 }
 
 // mc_polymer_move_arr, one lane per chain, positions read and written in
@@ -224,12 +237,12 @@ __device__ __forceinline__ void wave_count(uint32_t acc, unsigned long long* acc
 __global__ void __launch_bounds__(BLOCK) k_sweep_global(const KParams* __restrict__ Ps,
                                                         const BlockRef* __restrict__ bt) {
   const BlockRef br = block_ref(bt);
-  const KParams& kp = Ps[br.sys];
-  float* __restrict__ x = kp.x;
-  float* __restrict__ y = kp.y;
-  float* __restrict__ z = kp.z;
-  const float* __restrict__ omega = kp.omega;
-  const uint32_t sweep = kp.ctr[0];
+  KPC& kp = param(Ps, br.sys);
+  SCMF_GLOBAL float* __restrict__ x = gptr((float*)kp.x);
+  SCMF_GLOBAL float* __restrict__ y = gptr((float*)kp.y);
+  SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
+  const SCMF_GLOBAL float* __restrict__ omega = gptr((const float*)kp.omega);
+  const uint32_t sweep = ((const uint32_t*)kp.ctr)[0];
   const int64_t chain = (int64_t)br.local * BLOCK + threadIdx.x;
   uint32_t acc = 0;
   if (chain < kp.n_chains) {
@@ -237,11 +250,10 @@ __global__ void __launch_bounds__(BLOCK) k_sweep_global(const KParams* __restric
     const int length = kp.len[p];
     const int64_t off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
     const GlobalChain c{x + off, y + off, z + off};
-    const uint32_t id = kp.slot_chain[chain];
+    const uint32_t id = gptr((const uint32_t*)kp.slot_chain)[chain];
     for (int i = 0; i < length; i++) acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, omega);
   }
-  wave_count(acc, kp.accepts);
-  finish_sweep_block(kp, br);
+  wave_count(acc, (unsigned long long*)kp.accepts);
 }
 
 // mc_polymer_move_arr with the wavefront's 64 chains staged in LDS: one lane
@@ -252,12 +264,12 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
                                                   const BlockRef* __restrict__ bt, int nmax) {
   extern __shared__ float lds[];
   const BlockRef br = block_ref(bt);
-  const KParams& kp = Ps[br.sys];
-  float* __restrict__ x = kp.x;
-  float* __restrict__ y = kp.y;
-  float* __restrict__ z = kp.z;
-  const float* __restrict__ omega = kp.omega;
-  const uint32_t sweep = kp.ctr[0];
+  KPC& kp = param(Ps, br.sys);
+  SCMF_GLOBAL float* __restrict__ x = gptr((float*)kp.x);
+  SCMF_GLOBAL float* __restrict__ y = gptr((float*)kp.y);
+  SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
+  const SCMF_GLOBAL float* __restrict__ omega = gptr((const float*)kp.omega);
+  const uint32_t sweep = ((const uint32_t*)kp.ctr)[0];
   const int lane = threadIdx.x;
   const int64_t chain = (int64_t)br.local * 64 + lane;
   uint32_t acc = 0;
@@ -274,7 +286,7 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
       sz[b * 64] = z[off + b];
     }
     const LdsChain<64> c{sx, sy, sz};
-    const uint32_t id = kp.slot_chain[chain];
+    const uint32_t id = gptr((const uint32_t*)kp.slot_chain)[chain];
     for (int i = 0; i < length; i++) acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, omega);
     for (int b = 0; b < length; ++b) {
       x[off + b] = sx[b * 64];
@@ -282,8 +294,7 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
       z[off + b] = sz[b * 64];
     }
   }
-  wave_count(acc, kp.accepts);
-  finish_sweep_block(kp, br);
+  wave_count(acc, (unsigned long long*)kp.accepts);
 }
 
 // mc_polymer_move_arr with L lanes per chain (G = 64/L chains per wavefront).
@@ -308,12 +319,12 @@ __global__ void __launch_bounds__(64) k_sweep_sched(const KParams* __restrict__ 
   constexpr int G = 64 / L;
   extern __shared__ float lds[];
   const BlockRef br = block_ref(bt);
-  const KParams& kp = Ps[br.sys];
-  float* __restrict__ x = kp.x;
-  float* __restrict__ y = kp.y;
-  float* __restrict__ z = kp.z;
-  const float* __restrict__ omega = kp.omega;
-  const uint32_t sweep = kp.ctr[0];
+  KPC& kp = param(Ps, br.sys);
+  SCMF_GLOBAL float* __restrict__ x = gptr((float*)kp.x);
+  SCMF_GLOBAL float* __restrict__ y = gptr((float*)kp.y);
+  SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
+  const SCMF_GLOBAL float* __restrict__ omega = gptr((const float*)kp.omega);
+  const uint32_t sweep = ((const uint32_t*)kp.ctr)[0];
   const int lane = threadIdx.x;
   const int slot = lane / L, r = lane % L;
   float* sx = lds;
@@ -334,7 +345,7 @@ __global__ void __launch_bounds__(64) k_sweep_sched(const KParams* __restrict__ 
     p = arch_of_chain(kp, chain);
     length = kp.len[p];
     off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
-    id = kp.slot_chain[chain];
+    id = gptr((const uint32_t*)kp.slot_chain)[chain];
   }
   // load the tile and draw the picks
   for (int b = r; b < length; b += L) {
@@ -398,8 +409,7 @@ __global__ void __launch_bounds__(64) k_sweep_sched(const KParams* __restrict__ 
     y[off + b] = sy[b * G + slot];
     z[off + b] = sz[b * G + slot];
   }
-  wave_count(acc, kp.accepts);
-  finish_sweep_block(kp, br);
+  wave_count(acc, (unsigned long long*)kp.accepts);
 }
 
 template <int L, typename I>
@@ -412,10 +422,10 @@ size_t sched_lds_bytes(int nmax) {
 __global__ void __launch_bounds__(BLOCK) k_zero_counts(const KParams* __restrict__ Ps,
                                                        const BlockRef* __restrict__ bt) {
   const BlockRef br = block_ref(bt);
-  const KParams& kp = Ps[br.sys];
+  KPC& kp = param(Ps, br.sys);
   const int64_t stride = (int64_t)br.nblk * BLOCK;
-  for (int64_t i = (int64_t)br.local * BLOCK + threadIdx.x; i < kp.count_words; i += stride)
-    kp.counts[i] = 0u;
+  SCMF_GLOBAL uint32_t* __restrict__ c = gptr((uint32_t*)kp.counts);
+  for (int64_t i = (int64_t)br.local * BLOCK + threadIdx.x; i < kp.count_words; i += stride) c[i] = 0u;
 }
 
 __global__ void __launch_bounds__(BLOCK) k_zero(uint32_t* __restrict__ p, int64_t n) {
@@ -427,28 +437,28 @@ __global__ void __launch_bounds__(BLOCK) k_zero(uint32_t* __restrict__ p, int64_
 __global__ void __launch_bounds__(BLOCK) k_grid(const KParams* __restrict__ Ps,
                                                 const BlockRef* __restrict__ bt) {
   const BlockRef br = block_ref(bt);
-  const KParams& kp = Ps[br.sys];
-  const float* __restrict__ x = kp.x;
-  const float* __restrict__ y = kp.y;
-  const float* __restrict__ z = kp.z;
-  uint32_t* __restrict__ words = kp.counts;
+  KPC& kp = param(Ps, br.sys);
+  const SCMF_GLOBAL float* __restrict__ x = gptr((const float*)kp.x);
+  const SCMF_GLOBAL float* __restrict__ y = gptr((const float*)kp.y);
+  const SCMF_GLOBAL float* __restrict__ z = gptr((const float*)kp.z);
+  SCMF_GLOBAL uint32_t* __restrict__ words = gptr((uint32_t*)kp.counts);
   const bool b32 = kp.density_bits == 32;
   const int64_t stride = (int64_t)br.nblk * BLOCK;
   for (int64_t g = (int64_t)br.local * BLOCK + threadIdx.x; g < kp.n_beads; g += stride) {
-    const uint32_t t = kp.btype[g];
+    const uint32_t t = gptr((const uint8_t*)kp.btype)[g];
     const uint32_t cell = bin_cell(kp, (double)x[g], (double)y[g], (double)z[g], t);
     if (b32) {
-      atomicAdd(&words[cell], 1u);
+      atomicAdd((uint32_t*)&words[cell], 1u);
     } else {
       const uint32_t sh = (cell & 1u) * 16u;
-      const uint32_t old = atomicAdd(&words[cell >> 1], 1u << sh);
-      if (((old >> sh) & 0xFFFFu) == 0xFFFFu) atomicOr(kp.overflow, 1u);
+      const uint32_t old = atomicAdd((uint32_t*)&words[cell >> 1], 1u << sh);
+      if (((old >> sh) & 0xFFFFu) == 0xFFFFu) atomicOr((uint32_t*)kp.overflow, 1u);
     }
   }
 }
 
-template <int BITS>
-__device__ __forceinline__ uint32_t load_count(const uint32_t* words, uint32_t idx) {
+template <int BITS, class P>
+__device__ __forceinline__ uint32_t load_count(const P* words, uint32_t idx) {
   if (BITS == 32) return words[idx];
   return (words[idx >> 1] >> ((idx & 1u) * 16u)) & 0xFFFFu;
 }
@@ -460,10 +470,10 @@ __device__ __forceinline__ uint32_t load_count(const uint32_t* words, uint32_t i
 __global__ void __launch_bounds__(BLOCK) k_omega(const KParams* __restrict__ Ps,
                                                  const BlockRef* __restrict__ bt, int cadence) {
   const BlockRef br = block_ref(bt);
-  const KParams& kp = Ps[br.sys];
-  if (cadence && !kp.omega_every_step && kp.ctr[0] % 10u == 0u) return;
-  const uint32_t* __restrict__ words = kp.counts;
-  float* __restrict__ omega = kp.omega;
+  KPC& kp = param(Ps, br.sys);
+  if (cadence && !kp.omega_every_step && ((const uint32_t*)kp.ctr)[0] % 10u == 0u) return;
+  const SCMF_GLOBAL uint32_t* __restrict__ words = gptr((const uint32_t*)kp.counts);
+  SCMF_GLOBAL float* __restrict__ omega = gptr((float*)kp.omega);
   const uint32_t C = kp.n_cells;
   const int T = kp.T;
   const bool b32 = kp.density_bits == 32;
@@ -940,6 +950,7 @@ int launch_sweep(scmf_sys* s) {
     void* args[] = {(void*)&kpp, (void*)&none, (void*)&s->nmax};
     HIP_TRY(hipLaunchKernel(v->fn, grid, dim3(64), args, s->lds_bytes, s->stream));
   }
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s->stream, (const KParams*)s->d_kp, 1);
   HIP_TRY(hipGetLastError());
   return SCMF_OK;
 }
@@ -1062,6 +1073,25 @@ int sort_chains(scmf_sys* s) {
   return SCMF_OK;
 }
 
+// The tiled gridding of one system whose parameters sit at Ps[sys] (the
+// counts must already be cleared).
+int launch_tiled_grid(scmf_sys* s, hipStream_t st, const KParams* Ps, int sys) {
+  const dim3 tiles((unsigned)s->geo.ntiles);
+  if (s->opts.density_bits == 32) {
+    hipLaunchKernelGGL(k_grid_tiles<32>, tiles, dim3(1024), s->tile_lds, st, Ps, sys, s->geo,
+                       (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
+    hipLaunchKernelGGL(k_tiles_sum<32>, dim3(grid_strided(s->count_words)), dim3(256), 0, st, Ps,
+                       sys, s->geo, (const uint32_t*)s->scratch);
+  } else {
+    hipLaunchKernelGGL(k_grid_tiles<16>, tiles, dim3(1024), s->tile_lds, st, Ps, sys, s->geo,
+                       (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
+    hipLaunchKernelGGL(k_tiles_sum<16>, dim3(grid_strided(s->count_words)), dim3(256), 0, st, Ps,
+                       sys, s->geo, (const uint32_t*)s->scratch);
+  }
+  HIP_TRY(hipGetLastError());
+  return SCMF_OK;
+}
+
 int free_all(scmf_sys* s) {
   for (void* p : {(void*)s->x, (void*)s->y, (void*)s->z, (void*)s->counts, (void*)s->omega,
                   (void*)s->d_monotype, (void*)s->d_nbr_start, (void*)s->d_nbr,
@@ -1137,7 +1167,7 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
       (rc = alloc((void**)&s->z, sizeof(float) * B)) ||
       (rc = alloc((void**)&s->counts, sizeof(uint32_t) * s->count_words)) ||
       (rc = alloc((void**)&s->omega, sizeof(float) * TC)) ||
-      (rc = alloc((void**)&s->d_accepts, sizeof(unsigned long long))) ||
+      (rc = alloc((void**)&s->d_accepts, sizeof(unsigned long long) * ACC_SLOTS)) ||
       (rc = alloc((void**)&s->d_overflow, sizeof(uint32_t))) ||
       (rc = alloc((void**)&s->d_partial, sizeof(double) * grid_for(s->n_chains))))
     return bail(rc);
@@ -1174,7 +1204,7 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
   choose_tiles(s);
   s->sort_every = o.sort_every == 0 ? (s->tile_lds ? 25 : -1) : o.sort_every;
   if (hipMemsetAsync(s->d_fallback, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
-      hipMemsetAsync(s->d_accepts, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
+      hipMemsetAsync(s->d_accepts, 0, sizeof(unsigned long long) * ACC_SLOTS, s->stream) != hipSuccess ||
       hipMemsetAsync(s->d_overflow, 0, sizeof(uint32_t), s->stream) != hipSuccess ||
       hipMemsetAsync(s->omega, 0, sizeof(float) * TC, s->stream) != hipSuccess ||
       hipMemsetAsync(s->counts, 0, sizeof(uint32_t) * s->count_words, s->stream) != hipSuccess)
@@ -1283,23 +1313,8 @@ int scmf_grid_density(scmf_sys* s) {
                      (const KParams*)s->d_kp, (const BlockRef*)nullptr);
   const bool b32 = s->opts.density_bits == 32;
   if (s->tile_lds && s->sorted) {
-    const dim3 tiles((unsigned)s->geo.ntiles);
-    if (b32)
-      hipLaunchKernelGGL(k_grid_tiles<32>, tiles, dim3(1024), s->tile_lds, s->stream, s->kp,
-                         s->geo, s->x, s->y, s->z, s->btype, s->tstart, s->scratch, s->counts,
-                         s->d_overflow, s->d_fallback);
-    else
-      hipLaunchKernelGGL(k_grid_tiles<16>, tiles, dim3(1024), s->tile_lds, s->stream, s->kp,
-                         s->geo, s->x, s->y, s->z, s->btype, s->tstart, s->scratch, s->counts,
-                         s->d_overflow, s->d_fallback);
-    if (b32)
-      hipLaunchKernelGGL(k_tiles_sum<32>, dim3(grid_strided(s->count_words)), dim3(256), 0,
-                         s->stream, s->kp, s->geo, s->scratch, s->counts, s->d_overflow,
-                         s->count_words);
-    else
-      hipLaunchKernelGGL(k_tiles_sum<16>, dim3(grid_strided(s->count_words)), dim3(256), 0,
-                         s->stream, s->kp, s->geo, s->scratch, s->counts, s->d_overflow,
-                         s->count_words);
+    int rc = launch_tiled_grid(s, s->stream, s->d_kp, 0);
+    if (rc) return rc;
   } else {
     hipLaunchKernelGGL(k_grid, dim3(grid_strided(s->n_beads)), dim3(BLOCK), 0, s->stream,
                        (const KParams*)s->d_kp, (const BlockRef*)nullptr);
@@ -1449,15 +1464,19 @@ int scmf_stats(scmf_sys* s, scmf_stats_t* out) {
                      s->d_partial);
   HIP_TRY(hipGetLastError());
   std::vector<double> part(nb);
-  unsigned long long acc = 0, fb = 0;
+  unsigned long long fb = 0;
+  std::vector<unsigned long long> accs(ACC_SLOTS);
   uint32_t ovf = 0;
   HIP_TRY(hipMemcpyAsync(part.data(), s->d_partial, sizeof(double) * nb, hipMemcpyDeviceToHost, s->stream));
-  HIP_TRY(hipMemcpyAsync(&acc, s->d_accepts, sizeof acc, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipMemcpyAsync(accs.data(), s->d_accepts, sizeof(unsigned long long) * ACC_SLOTS,
+                         hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipMemcpyAsync(&ovf, s->d_overflow, sizeof ovf, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipMemcpyAsync(&fb, s->d_fallback, sizeof fb, hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   double sum = 0;
   for (double v : part) sum += v;
+  unsigned long long acc = 0;
+  for (unsigned long long v : accs) acc += v;
   out->moves = s->moves;
   out->accepts = acc;
   out->sweeps = s->sweeps;
@@ -1470,7 +1489,7 @@ int scmf_stats(scmf_sys* s, scmf_stats_t* out) {
 
 int scmf_reset_stats(scmf_sys* s) {
   if (!s) return fail(SCMF_E_ARG, "null handle");
-  HIP_TRY(hipMemsetAsync(s->d_accepts, 0, sizeof(unsigned long long), s->stream));
+  HIP_TRY(hipMemsetAsync(s->d_accepts, 0, sizeof(unsigned long long) * ACC_SLOTS, s->stream));
   HIP_TRY(hipMemsetAsync(s->d_overflow, 0, sizeof(uint32_t), s->stream));
   HIP_TRY(hipMemsetAsync(s->d_fallback, 0, sizeof(unsigned long long), s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -1517,8 +1536,9 @@ int scmf_autotune(scmf_sys* s, int32_t sweeps_per_variant, int32_t* chosen, floa
     (void)hipGetLastError();
     return fail(SCMF_E_NOMEM, "autotune needs a copy of the positions");
   }
-  unsigned long long acc0 = 0;
-  HIP_TRY(hipMemcpyAsync(&acc0, s->d_accepts, sizeof acc0, hipMemcpyDeviceToHost, s->stream));
+  std::vector<unsigned long long> acc0(ACC_SLOTS);
+  HIP_TRY(hipMemcpyAsync(acc0.data(), s->d_accepts, sizeof(unsigned long long) * ACC_SLOTS,
+                         hipMemcpyDeviceToHost, s->stream));
   HIP_TRY(hipMemcpyAsync(backup, s->x, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
   HIP_TRY(hipMemcpyAsync(backup + B, s->y, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
   HIP_TRY(hipMemcpyAsync(backup + 2 * B, s->z, sizeof(float) * B, hipMemcpyDeviceToDevice, s->stream));
@@ -1556,7 +1576,8 @@ int scmf_autotune(scmf_sys* s, int32_t sweeps_per_variant, int32_t* chosen, floa
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  HIP_TRY(hipMemcpyAsync(s->d_accepts, &acc0, sizeof acc0, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->d_accepts, acc0.data(), sizeof(unsigned long long) * ACC_SLOTS,
+                         hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   (void)hipFree(backup);
   s->sweeps = sweeps0;
@@ -1676,6 +1697,264 @@ int scmf_density_to_vtk(const char* density_path, const char* vtk_path) {
   std::string e = density_to_vtk(density_path, vtk_path);
   if (!e.empty()) return fail(SCMF_E_IO, e);
   return SCMF_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== batches
+// A batch advances many systems together: every phase of a step is one launch
+// (per sweep-kernel shape) over the workgroups of all members, each workgroup
+// looking up its member in a block table; the whole step is captured once in
+// a HIP graph and replayed.  Members keep their own memory and can still be
+// read or written through their handles between batch calls.
+struct scmf_batch {
+  std::vector<scmf_sys*> m;
+  hipStream_t stream = nullptr;
+  KParams* d_Ps = nullptr;
+  struct Group {
+    const SweepVariant* v;
+    int nmax;
+    size_t lds;
+    BlockRef* d_bt;
+    unsigned nblocks;
+  };
+  std::vector<Group> groups;
+  BlockRef* d_bt_zero = nullptr;
+  BlockRef* d_bt_grid = nullptr;
+  BlockRef* d_bt_omega = nullptr;
+  unsigned n_zero = 0, n_grid = 0, n_omega = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int forced_L = 0;  // 0: each member's own shape
+  uint64_t steps = 0;
+};
+
+namespace {
+
+int batch_free_tables(scmf_batch* b) {
+  for (auto& g : b->groups)
+    if (g.d_bt) (void)hipFree(g.d_bt);
+  b->groups.clear();
+  for (BlockRef** p : {&b->d_bt_zero, &b->d_bt_grid, &b->d_bt_omega})
+    if (*p) {
+      (void)hipFree(*p);
+      *p = nullptr;
+    }
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  return SCMF_OK;
+}
+
+int upload_table(const std::vector<BlockRef>& t, BlockRef** d) {
+  HIP_TRY(hipMalloc(d, sizeof(BlockRef) * std::max<size_t>(1, t.size())));
+  if (!t.empty()) HIP_TRY(hipMemcpy(*d, t.data(), sizeof(BlockRef) * t.size(), hipMemcpyHostToDevice));
+  return SCMF_OK;
+}
+
+// Workgroups for a grid-stride phase over `work` items: ~8 items per lane.
+uint32_t stride_blocks(int64_t work) {
+  int64_t nb = (work + BLOCK * 8 - 1) / (BLOCK * 8);
+  return (uint32_t)std::max<int64_t>(1, std::min<int64_t>(nb, 1 << 16));
+}
+
+// Build the parameter array, block tables and the step graph.
+int batch_build(scmf_batch* b) {
+  batch_free_tables(b);
+  const int n = (int)b->m.size();
+  std::vector<KParams> ps(n);
+  for (int i = 0; i < n; ++i) ps[i] = b->m[i]->kp;
+  HIP_TRY(hipMemcpy(b->d_Ps, ps.data(), sizeof(KParams) * n, hipMemcpyHostToDevice));
+  // sweep groups by kernel shape; members ordered by decreasing work first
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(),
+            [&](int a, int c) { return b->m[a]->n_beads > b->m[c]->n_beads; });
+  std::vector<std::vector<BlockRef>> tabs;
+  for (int i : order) {
+    scmf_sys* s = b->m[i];
+    const SweepVariant* v = s->variant;
+    if (b->forced_L) {
+      const int32_t keep = s->opts.lanes_per_chain;
+      s->opts.lanes_per_chain = b->forced_L;
+      const SweepVariant* keepv = s->variant;
+      const size_t keepl = s->lds_bytes;
+      if (choose_sweep(s) == SCMF_OK) v = s->variant;
+      s->variant = keepv;
+      s->lds_bytes = keepl;
+      s->opts.lanes_per_chain = keep;
+    }
+    int gi = -1;
+    for (size_t k = 0; k < b->groups.size(); ++k)
+      if (b->groups[k].v == v) gi = (int)k;
+    if (gi < 0) {
+      b->groups.push_back({v, 0, 0, nullptr, 0});
+      tabs.emplace_back();
+      gi = (int)b->groups.size() - 1;
+    }
+    scmf_batch::Group& g = b->groups[gi];
+    g.nmax = std::max(g.nmax, s->nmax);
+    const int G = v->L == 0 ? BLOCK : 64 / v->L;
+    const uint32_t nb = (uint32_t)((s->n_chains + G - 1) / G);
+    for (uint32_t k = 0; k < nb; ++k) tabs[gi].push_back({(uint32_t)i, k, nb, 0});
+  }
+  for (size_t k = 0; k < b->groups.size(); ++k) {
+    scmf_batch::Group& g = b->groups[k];
+    g.lds = g.v->bytes(g.nmax);
+    if (g.lds > 65536 &&
+        hipFuncSetAttribute(g.v->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.lds) !=
+            hipSuccess) {
+      (void)hipGetLastError();
+      return fail(SCMF_E_ARG, "batch members' chains are too long for one LDS tile shape");
+    }
+    g.nblocks = (unsigned)tabs[k].size();
+    int rc = upload_table(tabs[k], &g.d_bt);
+    if (rc) return rc;
+  }
+  std::vector<BlockRef> tz, tg, to;
+  for (int i : order) {
+    scmf_sys* s = b->m[i];
+    const uint32_t nz = stride_blocks(s->count_words);
+    for (uint32_t k = 0; k < nz; ++k) tz.push_back({(uint32_t)i, k, nz, 0});
+    if (!(s->tile_lds && s->sorted)) {
+      const uint32_t ng = stride_blocks(s->n_beads);
+      for (uint32_t k = 0; k < ng; ++k) tg.push_back({(uint32_t)i, k, ng, 0});
+    }
+    const uint32_t no = stride_blocks(s->n_cells);
+    for (uint32_t k = 0; k < no; ++k) to.push_back({(uint32_t)i, k, no, 0});
+  }
+  int rc;
+  if ((rc = upload_table(tz, &b->d_bt_zero)) || (rc = upload_table(tg, &b->d_bt_grid)) ||
+      (rc = upload_table(to, &b->d_bt_omega)))
+    return rc;
+  b->n_zero = (unsigned)tz.size();
+  b->n_grid = (unsigned)tg.size();
+  b->n_omega = (unsigned)to.size();
+  // capture one step
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const KParams* Ps = b->d_Ps;
+  for (const scmf_batch::Group& g : b->groups) {
+    if (g.v->L == 0) {
+      hipLaunchKernelGGL(k_sweep_global, dim3(g.nblocks), dim3(BLOCK), 0, b->stream, Ps,
+                         (const BlockRef*)g.d_bt);
+    } else {
+      const BlockRef* bt = g.d_bt;
+      int nmax = g.nmax;
+      void* args[] = {(void*)&Ps, (void*)&bt, (void*)&nmax};
+      (void)hipLaunchKernel(g.v->fn, dim3(g.nblocks), dim3(64), args, g.lds, b->stream);
+    }
+  }
+  hipLaunchKernelGGL(k_advance, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, b->stream, Ps, n);
+  hipLaunchKernelGGL(k_zero_counts, dim3(b->n_zero), dim3(BLOCK), 0, b->stream, Ps,
+                     (const BlockRef*)b->d_bt_zero);
+  if (b->n_grid)
+    hipLaunchKernelGGL(k_grid, dim3(b->n_grid), dim3(BLOCK), 0, b->stream, Ps,
+                       (const BlockRef*)b->d_bt_grid);
+  for (int i = 0; i < n; ++i)
+    if (b->m[i]->tile_lds && b->m[i]->sorted) (void)launch_tiled_grid(b->m[i], b->stream, Ps, i);
+  hipLaunchKernelGGL(k_omega, dim3(b->n_omega), dim3(BLOCK), 0, b->stream, Ps,
+                     (const BlockRef*)b->d_bt_omega, 1);
+  hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+  if (e != hipSuccess) return fail(SCMF_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return SCMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scmf_batch_create(scmf_batch** out, scmf_sys* const* members, int32_t n_members) {
+  if (!out || !members || n_members <= 0) return fail(SCMF_E_ARG, "batch needs members");
+  *out = nullptr;
+  scmf_batch* b = new scmf_batch();
+  for (int32_t i = 0; i < n_members; ++i) {
+    if (!members[i]) {
+      delete b;
+      return fail(SCMF_E_ARG, "null batch member");
+    }
+    for (scmf_sys* o : b->m)
+      if (o == members[i]) {
+        delete b;
+        return fail(SCMF_E_ARG, "a system appears twice in the batch");
+      }
+    b->m.push_back(members[i]);
+  }
+  int rc = SCMF_OK;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&b->d_Ps, sizeof(KParams) * n_members) != hipSuccess) {
+    (void)hipGetLastError();
+    rc = fail(SCMF_E_HIP, "batch allocation failed");
+  }
+  for (scmf_sys* s : b->m) {
+    if (rc) break;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(SCMF_E_HIP, "member sync failed");
+    if (!rc && s->tile_lds && !s->sorted && s->sort_every > 0) rc = sort_chains(s);
+    if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(SCMF_E_HIP, "member sync failed");
+  }
+  if (!rc) rc = batch_build(b);
+  if (rc) {
+    scmf_batch_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return SCMF_OK;
+}
+
+void scmf_batch_destroy(scmf_batch* b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_free_tables(b);
+  if (b->d_Ps) (void)hipFree(b->d_Ps);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int scmf_batch_step(scmf_batch* b, int32_t n_steps) {
+  if (!b) return fail(SCMF_E_ARG, "null batch");
+  if (n_steps < 0) return fail(SCMF_E_ARG, "n_steps must be non-negative");
+  for (int32_t k = 0; k < n_steps; ++k) {
+    // spatial reordering of tile-gridded members, between graph replays
+    bool rebuilt = false;
+    for (scmf_sys* s : b->m) {
+      if (s->tile_lds && s->sort_every > 0 && s->sweeps % (uint64_t)s->sort_every == 0 &&
+          s->sweeps > 0) {
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        int rc = sort_chains(s);
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        rebuilt = true;
+      }
+    }
+    if (rebuilt) {
+      std::vector<KParams> ps(b->m.size());
+      for (size_t i = 0; i < b->m.size(); ++i) ps[i] = b->m[i]->kp;
+      HIP_TRY(hipMemcpyAsync(b->d_Ps, ps.data(), sizeof(KParams) * ps.size(),
+                             hipMemcpyHostToDevice, b->stream));
+      HIP_TRY(hipStreamSynchronize(b->stream));
+    }
+    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    for (scmf_sys* s : b->m) {
+      s->sweeps += 1;
+      s->moves += (uint64_t)s->n_beads;
+    }
+    b->steps += 1;
+  }
+  return SCMF_OK;
+}
+
+int scmf_batch_sync(scmf_batch* b) {
+  if (!b) return fail(SCMF_E_ARG, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return SCMF_OK;
+}
+
+int scmf_batch_set_lanes(scmf_batch* b, int32_t lanes_per_chain) {
+  if (!b) return fail(SCMF_E_ARG, "null batch");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->forced_L = lanes_per_chain;
+  return batch_build(b);
 }
 
 }  // extern "C"

@@ -43,9 +43,9 @@ __global__ void __launch_bounds__(256) k_tile_keys(KParams kp, TileGeo g,
   if (s >= kp.n_chains) return;
   const int p = arch_of_chain(kp, s);
   const int64_t a = kp.bead_base[p] + (s - kp.chain_base[p]) * kp.len[p] + kp.len[p] / 2;
-  const int xi = fold_fast((double)x[a], kp.ax[0]);
-  const int yi = fold_fast((double)y[a], kp.ax[1]);
-  const int zi = fold_fast((double)z[a], kp.ax[2]);
+  const int xi = fold_fast((double)x[a], axis(kp, 0));
+  const int yi = fold_fast((double)y[a], axis(kp, 1));
+  const int zi = fold_fast((double)z[a], axis(kp, 2));
   const uint32_t t = (uint32_t)(xi / g.ts[0] + g.nt[0] * (yi / g.ts[1] + g.nt[1] * (zi / g.ts[2])));
   const uint32_t key = (uint32_t)p * (uint32_t)g.ntiles + t;
   keys[s] = key;
@@ -146,17 +146,19 @@ __device__ __forceinline__ int wrap_rel(int c, int origin, int n) {
 // window, store the window.  Beads outside the window go straight to the
 // final counts with a global atomic.
 template <int BITS>
-__global__ void __launch_bounds__(1024) k_grid_tiles(KParams kp, TileGeo g,
-                                                     const float* __restrict__ x,
-                                                     const float* __restrict__ y,
-                                                     const float* __restrict__ z,
-                                                     const uint8_t* __restrict__ btype,
+__global__ void __launch_bounds__(1024) k_grid_tiles(const KParams* __restrict__ Ps, int sys,
+                                                     TileGeo g,
                                                      const uint32_t* __restrict__ tile_start,
                                                      uint32_t* __restrict__ scratch,
-                                                     uint32_t* __restrict__ counts,
-                                                     uint32_t* __restrict__ overflow,
                                                      unsigned long long* __restrict__ fallback) {
   extern __shared__ uint32_t win[];
+  KPC& kp = param(Ps, (uint32_t)sys);
+  const SCMF_GLOBAL float* __restrict__ x = gptr((const float*)kp.x);
+  const SCMF_GLOBAL float* __restrict__ y = gptr((const float*)kp.y);
+  const SCMF_GLOBAL float* __restrict__ z = gptr((const float*)kp.z);
+  const SCMF_GLOBAL uint8_t* __restrict__ btype = gptr((const uint8_t*)kp.btype);
+  SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
+  uint32_t* __restrict__ overflow = (uint32_t*)kp.overflow;
   const int t = blockIdx.x;
   for (int w = threadIdx.x; w < g.win_words; w += 1024) win[w] = 0u;
   __syncthreads();
@@ -170,9 +172,9 @@ __global__ void __launch_bounds__(1024) k_grid_tiles(KParams kp, TileGeo g,
     const int64_t b1 = kp.bead_base[p] + (s1 - kp.chain_base[p]) * len;
     for (int64_t gb = b0 + threadIdx.x; gb < b1; gb += 1024) {
       const uint32_t ty_ = btype[gb];
-      const int xi = fold_fast((double)x[gb], kp.ax[0]);
-      const int yi = fold_fast((double)y[gb], kp.ax[1]);
-      const int zi = fold_fast((double)z[gb], kp.ax[2]);
+      const int xi = fold_fast((double)x[gb], axis(kp, 0));
+      const int yi = fold_fast((double)y[gb], axis(kp, 1));
+      const int zi = fold_fast((double)z[gb], axis(kp, 2));
       const int wx = wrap_rel(xi, ox, kp.nx), wy = wrap_rel(yi, oy, kp.ny),
                 wz = wrap_rel(zi, oz, kp.nz);
       bool done = false;
@@ -192,10 +194,10 @@ __global__ void __launch_bounds__(1024) k_grid_tiles(KParams kp, TileGeo g,
       if (!done) {
         const uint32_t cell = ty_ * kp.n_cells + (uint32_t)(xi + kp.nx * (yi + kp.ny * zi));
         if (BITS == 32) {
-          atomicAdd(&counts[cell], 1u);
+          atomicAdd((uint32_t*)&counts[cell], 1u);
         } else {
           const uint32_t sh = (cell & 1u) * 16u;
-          const uint32_t old = atomicAdd(&counts[cell >> 1], 1u << sh);
+          const uint32_t old = atomicAdd((uint32_t*)&counts[cell >> 1], 1u << sh);
           if (((old >> sh) & 0xFFFFu) == 0xFFFFu) atomicOr(overflow, 1u);
         }
       }
@@ -240,11 +242,13 @@ __device__ __forceinline__ uint32_t window_count(const uint32_t* scratch, const 
 // atomics).  One lane per 32-bit word of counts: one cell (uint32) or a pair
 // of cells (uint16), so no two lanes write the same word.
 template <int BITS>
-__global__ void __launch_bounds__(256) k_tiles_sum(KParams kp, TileGeo g,
-                                                   const uint32_t* __restrict__ scratch,
-                                                   uint32_t* __restrict__ counts,
-                                                   uint32_t* __restrict__ overflow,
-                                                   int64_t n_words) {
+__global__ void __launch_bounds__(256) k_tiles_sum(const KParams* __restrict__ Ps, int sys,
+                                                   TileGeo g,
+                                                   const uint32_t* __restrict__ scratch) {
+  KPC& kp = param(Ps, (uint32_t)sys);
+  SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
+  uint32_t* __restrict__ overflow = (uint32_t*)kp.overflow;
+  const int64_t n_words = kp.count_words;
   const int64_t stride = (int64_t)gridDim.x * 256;
   for (int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x; w < n_words; w += stride) {
     uint32_t out = 0;

@@ -58,6 +58,12 @@ def lib():
     L.scmf_write_density_host.argtypes = [C.c_char_p, C.POINTER(C.c_int32),
                                           C.POINTER(C.c_double), C.c_int32, vp, vp, C.c_double]
     L.scmf_autotune.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_float)]
+    L.scmf_batch_create.argtypes = [C.POINTER(vp), C.POINTER(vp), C.c_int32]
+    L.scmf_batch_step.argtypes = [vp, C.c_int32]
+    L.scmf_batch_sync.argtypes = [vp]
+    L.scmf_batch_set_lanes.argtypes = [vp, C.c_int32]
+    L.scmf_batch_destroy.argtypes = [vp]
+    L.scmf_batch_destroy.restype = None
     L.scmf_last_error.restype = C.c_char_p
     L.scmf_version.restype = C.c_char_p
     _lib = L
@@ -210,3 +216,39 @@ class System:
 
     def write_coord(self, path):
         _check(lib().scmf_write_coord(self._h, str(path).encode()))
+
+
+class Batch:
+    """Many systems advanced together by captured one-launch-per-phase steps."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m._h.value for m in self.members])
+        self._h = C.c_void_p()
+        _check(lib().scmf_batch_create(C.byref(self._h), arr, len(self.members)))
+
+    def step(self, n=1):
+        _check(lib().scmf_batch_step(self._h, int(n)))
+
+    def sync(self):
+        _check(lib().scmf_batch_sync(self._h))
+
+    def set_lanes(self, lanes_per_chain):
+        _check(lib().scmf_batch_set_lanes(self._h, int(lanes_per_chain)))
+
+    def close(self):
+        if self._h:
+            lib().scmf_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

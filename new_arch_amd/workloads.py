@@ -89,6 +89,35 @@ def mixed_arch(scale=1) -> SystemSpec:
                               monotype=[0] + [1, 1, 2] * 4, bonds=star)])
 
 
+def batched_mix(n_systems=64, seed=0, sizes=(16, 24, 32, 48, 64), lengths=(16, 24, 32, 48, 64),
+                density_per_re3=128.0):
+    """Independent systems of uneven size for batched runs: AB diblock melts or
+    A/B homopolymer blends, N from `lengths`, grid edge from `sizes` (cells of
+    Re/8), chiN uniform in [10, 40], kappaN = 50, fA uniform in [0.2, 0.8]
+    (A fraction of each diblock, or of the chains in a blend), chain counts
+    giving ~`density_per_re3` beads per Re^3.  Returns (spec, seed, system_id)
+    triples; the streams of system i are keyed by (1000 seed + i, i)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_systems):
+        N = int(rng.choice(lengths))
+        g = int(rng.choice(sizes))
+        chi = float(rng.uniform(10.0, 40.0))
+        fA = float(rng.uniform(0.2, 0.8))
+        L = g / 8.0
+        chains = max(1, int(round(density_per_re3 * L ** 3 / N)))
+        if rng.uniform() < 0.5:
+            polys = [diblock(N, fA, chains)]
+        else:
+            na = max(1, int(round(fA * chains)))
+            polys = [homopolymer(N, na, 0)] + ([homopolymer(N, chains - na, 1)]
+                                               if chains > na else [])
+        spec = SystemSpec(n_types=2, xn=xn_matrix(2, 50.0, chi), L=[L, L, L], n=[g, g, g],
+                          dtNxi=0.17, nref=N, polymers=polys)
+        out.append((spec, seed * 1000 + i, i))
+    return out
+
+
 NAMED = {
     "small_diblock_melt": small_diblock_melt,
     "large_diblock_melt": large_diblock_melt,
