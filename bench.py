@@ -135,6 +135,56 @@ def batched_vs_plain(steps, warmup, n_systems=64):
             "batched_ms_per_step": t_batch * 1e3 / steps}
 
 
+def served(n_jobs, rate, seed=0, steps_per_tick=25):
+    """Poisson stream of jobs into the continuous-batching server, driven in
+    wall-clock time: p50/p99 latency (finish - arrival) and aggregate
+    bead-moves/s over the run."""
+    import numpy as np
+
+    from new_arch_amd import scmf
+    from new_arch_amd import workloads as W
+    from new_arch_amd.abi import JOB_DONE, make_opts
+
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_jobs))
+    js = [W.served_job(rng) for _ in range(n_jobs)]
+    sv = scmf.Server(steps_per_tick=steps_per_tick)
+    t0 = sv.clock()
+    submitted, live, lat, moves = 0, {}, [], 0
+    t_first = None
+    while len(lat) < n_jobs:
+        now = sv.clock() - t0
+        while submitted < n_jobs and arrivals[submitted] <= now:
+            spec, sweeps = js[submitted]
+            jid = sv.submit(spec, make_opts(seed=10_000 + submitted, system_id=submitted,
+                                            omega_every_step=True, density_bits=16), sweeps)
+            live[jid] = (spec.n_beads, sweeps)
+            t_first = t_first if t_first is not None else sv.info(jid).t_submit
+            submitted += 1
+        q, r = sv.tick()
+        for jid in [j for j in live if sv.info(j).state == JOB_DONE]:
+            info = sv.info(jid)
+            lat.append(info.t_done - info.t_submit)
+            beads, sweeps = live.pop(jid)
+            moves += beads * sweeps
+            t_last = info.t_done
+            sv.release(jid)
+        if q == 0 and r == 0 and submitted < n_jobs:
+            time.sleep(max(0.0, min(0.01, arrivals[submitted] - (sv.clock() - t0))))
+    sv.close()
+    lat = np.array(lat)
+    agg = moves / (t_last - t_first)
+    log(f"served: {n_jobs} jobs at {rate}/s; p50 {np.percentile(lat, 50):.3f} s, "
+        f"p99 {np.percentile(lat, 99):.3f} s, {agg:.3e} bead-moves/s")
+    return {"workload": "served_jobs", "jobs": n_jobs, "arrival_rate_per_s": rate,
+            "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_latency_s": float(lat.mean()), "aggregate_bead_moves_per_s": agg,
+            "moves": int(moves), "span_s": t_last - t_first,
+            "job_mix": "AB diblock or ABA triblock melts, N 16-128, grid 16^3-128^3 (Re/8 "
+                       "cells), chiN 10-50, 500-5000 sweeps, 128 beads/Re^3, uint16"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +193,9 @@ def main():
     ap.add_argument("--workload", default="large_diblock_melt")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-jobs", type=int, default=200)
+    ap.add_argument("--served-rate", type=float, default=6.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs one system per GPU and measures a single GPU; "
@@ -228,6 +281,9 @@ def main():
     if not args.no_batched:
         log("batched vs plain")
         out["batched"] = batched_vs_plain(steps=50, warmup=5)
+    if not args.no_served:
+        log("served")
+        out["served"] = served(args.served_jobs, args.served_rate)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(spec, opts)

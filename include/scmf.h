@@ -211,6 +211,47 @@ int scmf_batch_sync(scmf_batch* batch);
 int scmf_batch_set_lanes(scmf_batch* batch, int32_t lanes_per_chain);
 void scmf_batch_destroy(scmf_batch* batch);
 
+/* ---- serving: jobs submitted at any time, run by continuous batching -------
+ * No counterpart in the reference.  A job is a system (description, options)
+ * and a sweep count.  scmf_server_tick admits queued jobs in arrival order
+ * while the device-memory budget and the slot count allow, runs every admitted
+ * job through one batch (sweep, grid, omega per step, as scmf_step), and marks
+ * the jobs that reached their sweep count DONE; their state stays resident
+ * until released.  A finished job's result equals running it alone. */
+#define SCMF_JOB_QUEUED 0
+#define SCMF_JOB_RUNNING 1
+#define SCMF_JOB_DONE 2
+#define SCMF_JOB_RELEASED 3
+#define SCMF_JOB_FAILED 4
+
+typedef struct scmf_server_opts {
+  uint64_t hbm_budget;    /* device bytes the jobs may hold (0: 200 GiB) */
+  int32_t max_jobs;       /* running jobs at most (0: 4096) */
+  int32_t steps_per_tick; /* sweeps per tick at most (0: 50) */
+} scmf_server_opts;
+
+typedef struct scmf_job_info {
+  int32_t state;
+  int32_t pad;
+  int64_t sweeps_done, sweeps_target;
+  double t_submit, t_admit, t_done; /* seconds on the server clock */
+  uint64_t device_bytes;            /* admission estimate */
+  int64_t beads;
+} scmf_job_info;
+
+typedef struct scmf_server scmf_server;
+int scmf_server_create(scmf_server** out, const scmf_server_opts* opts);
+int scmf_server_submit(scmf_server* server, const scmf_desc* desc, const scmf_opts* opts,
+                       int32_t n_sweeps, uint64_t* job_id);
+int scmf_server_tick(scmf_server* server, int32_t* n_queued, int32_t* n_running);
+int scmf_server_info(scmf_server* server, uint64_t job_id, scmf_job_info* out);
+/* Handle of a finished job, for scmf_copy_out / scmf_stats / the writers. */
+int scmf_server_job_system(scmf_server* server, uint64_t job_id, scmf_sys** out);
+int scmf_server_release(scmf_server* server, uint64_t job_id);
+double scmf_server_clock(scmf_server* server);
+const char* scmf_server_last_error(void);
+void scmf_server_destroy(scmf_server* server);
+
 const char* scmf_last_error(void);
 const char* scmf_version(void);
 

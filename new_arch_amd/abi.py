@@ -95,6 +95,20 @@ class Summary(C.Structure):
     ]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_jobs", C.c_int32),
+                ("steps_per_tick", C.c_int32)]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_RELEASED, JOB_FAILED = range(5)
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("pad", C.c_int32), ("sweeps_done", C.c_int64),
+                ("sweeps_target", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("device_bytes", C.c_uint64), ("beads", C.c_int64)]
+
+
 def linear_bonds(n: int) -> np.ndarray:
     """Bond matrix of a linear chain, as polymer.dat stores it (row i, column j)."""
     m = np.zeros((n, n), dtype=np.uint8)

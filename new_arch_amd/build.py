@@ -33,7 +33,8 @@ def build(verbose=True):
     os.makedirs(LIBDIR, exist_ok=True)
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, "-shared", "-o", tmp,
-          os.path.join(CSRC, "scmf_engine.hip"), os.path.join(CSRC, "scmf_host.cpp")])
+          os.path.join(CSRC, "scmf_engine.hip"), os.path.join(CSRC, "scmf_host.cpp"),
+          os.path.join(CSRC, "scmf_server.cpp")])
     os.replace(tmp, LIB)
     tmp = RUNNER + ".tmp"
     _run([HIPCC, *COMMON, "-o", tmp, os.path.join(CSRC, "scmf_main.cpp"),

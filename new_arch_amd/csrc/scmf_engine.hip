@@ -1202,6 +1202,9 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
                      s->chain_slot, s->n_chains);
   hipLaunchKernelGGL(k_fill_btype, dim3(grid_strided(B)), dim3(BLOCK), 0, s->stream, s->kp, s->btype);
   choose_tiles(s);
+  // Automatic ordering only where global atomics would cost more than the
+  // reordering (below ~8M beads the atomic gridding takes < 0.3 ms).
+  if (o.sort_every == 0 && s->n_beads < ((int64_t)1 << 23)) s->tile_lds = 0;
   s->sort_every = o.sort_every == 0 ? (s->tile_lds ? 25 : -1) : o.sort_every;
   if (hipMemsetAsync(s->d_fallback, 0, sizeof(unsigned long long), s->stream) != hipSuccess ||
       hipMemsetAsync(s->d_accepts, 0, sizeof(unsigned long long) * ACC_SLOTS, s->stream) != hipSuccess ||

@@ -8,8 +8,8 @@ import os
 
 import numpy as np
 
-from .abi import (DENSITY, OMEGA, POS, SCMF_OK, Desc, Opts, Stats, Summary, SystemSpec,
-                  Timing, make_opts)
+from .abi import (DENSITY, OMEGA, POS, SCMF_OK, Desc, JobInfo, Opts, ServerOpts, Stats,
+                  Summary, SystemSpec, Timing, make_opts)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libscmf.so")
@@ -64,6 +64,18 @@ def lib():
     L.scmf_batch_set_lanes.argtypes = [vp, C.c_int32]
     L.scmf_batch_destroy.argtypes = [vp]
     L.scmf_batch_destroy.restype = None
+    L.scmf_server_create.argtypes = [C.POINTER(vp), C.POINTER(ServerOpts)]
+    L.scmf_server_submit.argtypes = [vp, C.POINTER(Desc), C.POINTER(Opts), C.c_int32,
+                                     C.POINTER(C.c_uint64)]
+    L.scmf_server_tick.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.scmf_server_info.argtypes = [vp, C.c_uint64, C.POINTER(JobInfo)]
+    L.scmf_server_job_system.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+    L.scmf_server_release.argtypes = [vp, C.c_uint64]
+    L.scmf_server_clock.argtypes = [vp]
+    L.scmf_server_clock.restype = C.c_double
+    L.scmf_server_last_error.restype = C.c_char_p
+    L.scmf_server_destroy.argtypes = [vp]
+    L.scmf_server_destroy.restype = None
     L.scmf_last_error.restype = C.c_char_p
     L.scmf_version.restype = C.c_char_p
     _lib = L
@@ -246,6 +258,76 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _scheck(rc):
+    if rc != SCMF_OK:
+        raise ScmfError(rc, lib().scmf_server_last_error().decode())
+
+
+class JobView(System):
+    """A finished job's resident state, borrowed from the server (not owned)."""
+
+    def __init__(self, handle, spec):
+        self._h = C.c_void_p(handle)
+        self.spec = spec
+        nb, nc, ncell, nt = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        _check(lib().scmf_sizes(self._h, C.byref(nb), C.byref(nc), C.byref(ncell), C.byref(nt)))
+        self.n_beads, self.n_chains, self.n_cells, self.n_types = (nb.value, nc.value,
+                                                                   ncell.value, nt.value)
+
+    def close(self):
+        self._h = C.c_void_p()
+
+
+class Server:
+    """Continuous-batching job server (see include/scmf.h)."""
+
+    def __init__(self, hbm_budget=0, max_jobs=0, steps_per_tick=0):
+        o = ServerOpts(int(hbm_budget), int(max_jobs), int(steps_per_tick))
+        self._h = C.c_void_p()
+        _scheck(lib().scmf_server_create(C.byref(self._h), C.byref(o)))
+        self.specs = {}
+
+    def submit(self, spec: SystemSpec, opts: Opts, sweeps: int) -> int:
+        d, keep = spec.to_desc()
+        jid = C.c_uint64()
+        _scheck(lib().scmf_server_submit(self._h, C.byref(d), C.byref(opts), int(sweeps),
+                                         C.byref(jid)))
+        self.specs[jid.value] = spec
+        return jid.value
+
+    def tick(self):
+        q, r = C.c_int32(), C.c_int32()
+        _scheck(lib().scmf_server_tick(self._h, C.byref(q), C.byref(r)))
+        return q.value, r.value
+
+    def info(self, jid) -> JobInfo:
+        i = JobInfo()
+        _scheck(lib().scmf_server_info(self._h, int(jid), C.byref(i)))
+        return i
+
+    def job(self, jid) -> JobView:
+        h = C.c_void_p()
+        _scheck(lib().scmf_server_job_system(self._h, int(jid), C.byref(h)))
+        return JobView(h.value, self.specs.get(jid))
+
+    def release(self, jid):
+        _scheck(lib().scmf_server_release(self._h, int(jid)))
+
+    def clock(self) -> float:
+        return lib().scmf_server_clock(self._h)
+
+    def close(self):
+        if self._h:
+            lib().scmf_server_destroy(self._h)
+            self._h = C.c_void_p()
 
     def __del__(self):
         try:

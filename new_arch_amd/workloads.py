@@ -118,6 +118,30 @@ def batched_mix(n_systems=64, seed=0, sizes=(16, 24, 32, 48, 64), lengths=(16, 2
     return out
 
 
+def triblock(N: int, fA: float, count: int) -> PolymerType:
+    """ABA triblock: A ends of fA/2 each around a B middle block."""
+    na = int(round(N * fA / 2))
+    return PolymerType(length=N, count=count, monotype=[0] * na + [1] * (N - 2 * na) + [0] * na)
+
+
+def served_job(rng, density_per_re3=128.0, n_range=(16, 128), grid_range=(16, 128),
+               chi_range=(10.0, 50.0), sweep_range=(500, 5000)):
+    """One serving job: an AB diblock or ABA triblock melt with N, grid edge
+    (cells of Re/8), chiN and sweep count drawn uniformly from the ranges,
+    kappaN = 50, fA uniform in [0.2, 0.8], ~`density_per_re3` beads per Re^3.
+    Returns (spec, sweeps)."""
+    N = int(rng.integers(n_range[0], n_range[1] + 1))
+    g = int(rng.integers(grid_range[0], grid_range[1] + 1))
+    chi = float(rng.uniform(*chi_range))
+    fA = float(rng.uniform(0.2, 0.8))
+    L = g / 8.0
+    chains = max(1, int(round(density_per_re3 * L ** 3 / N)))
+    poly = diblock(N, fA, chains) if rng.uniform() < 0.5 else triblock(N, fA, chains)
+    spec = SystemSpec(n_types=2, xn=xn_matrix(2, 50.0, chi), L=[L, L, L], n=[g, g, g],
+                      dtNxi=0.17, nref=N, polymers=[poly])
+    return spec, int(rng.integers(sweep_range[0], sweep_range[1] + 1))
+
+
 NAMED = {
     "small_diblock_melt": small_diblock_melt,
     "large_diblock_melt": large_diblock_melt,

@@ -1,0 +1,99 @@
+"""Serving: jobs of mixed size and length submitted over time, admitted
+against a memory budget, run by continuous batching; every finished job equals
+the same job run alone (bit for bit), and memory bookkeeping holds."""
+import numpy as np
+import pytest
+
+from new_arch_amd import scmf
+from new_arch_amd import workloads as W
+from new_arch_amd.abi import JOB_DONE, JOB_QUEUED, JOB_RELEASED, make_opts
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-server] {msg}", flush=True)
+
+
+def same(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint32),
+                          np.ascontiguousarray(b).view(np.uint32))
+
+
+def jobs(n, seed=11):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        spec, _ = W.served_job(rng, n_range=(12, 40), grid_range=(12, 28))
+        sweeps = int(rng.integers(3, 30))
+        out.append((spec, make_opts(seed=100 + k, system_id=k, omega_every_step=bool(k % 2),
+                                    density_bits=16), sweeps))
+    return out
+
+
+def alone(spec, opts, sweeps):
+    g = scmf.System(spec, opts)
+    g.grid_density()
+    g.update_omega()
+    for step in range(1, sweeps + 1):
+        g.step(step)
+    return g
+
+
+def test_jobs_equal_alone_under_continuous_batching():
+    banner("jobs vs alone")
+    js = jobs(10)
+    sv = scmf.Server(steps_per_tick=7)
+    ids = []
+    for k, (spec, opts, sweeps) in enumerate(js):
+        ids.append(sv.submit(spec, opts, sweeps))
+        if k % 3 == 2:                   # arrivals while others are running
+            sv.tick()
+    for _ in range(200):
+        q, r = sv.tick()
+        if q == 0 and r == 0:
+            break
+    for jid, (spec, opts, sweeps) in zip(ids, js):
+        info = sv.info(jid)
+        assert info.state == JOB_DONE and info.sweeps_done == sweeps
+        assert info.t_done >= info.t_admit >= info.t_submit
+        got = sv.job(jid)
+        ref = alone(spec, opts, sweeps)
+        assert same(got.positions(), ref.positions()), f"job {jid}"
+        np.testing.assert_array_equal(got.density(), ref.density())
+        assert same(got.omega(), ref.omega())
+        assert got.stats().accepts == ref.stats().accepts
+        ref.close()
+        sv.release(jid)
+        assert sv.info(jid).state == JOB_RELEASED
+    sv.close()
+
+
+def test_admission_against_budget():
+    banner("admission")
+    js = jobs(6, seed=5)
+    def est(spec):                        # the server's admission estimate
+        s = scmf.inspect_spec(spec)
+        return (s.device_bytes_16 + 12 * s.n_beads + 16 * s.n_chains + 64 * s.n_cells
+                + (1 << 20))
+    need = max(est(spec) for spec, _, _ in js)
+    sv = scmf.Server(hbm_budget=int(need * 2.5), steps_per_tick=5)
+    ids = [sv.submit(spec, opts, sweeps) for spec, opts, sweeps in js]
+    q, r = sv.tick()
+    assert r >= 1 and q + r <= len(js)
+    seen_queued = q > 0
+    for _ in range(300):
+        q, r = sv.tick()
+        for jid in ids:                   # finished jobs give their memory back
+            if sv.info(jid).state == JOB_DONE:
+                sv.release(jid)
+        if q == 0 and r == 0:
+            break
+    assert all(sv.info(j).state == JOB_RELEASED for j in ids)
+    assert seen_queued
+    spec, opts, _ = js[0]
+    tiny = scmf.Server(hbm_budget=1024)
+    with pytest.raises(scmf.ScmfError):
+        tiny.submit(spec, opts, 5)
+    tiny.close()
+    sv.close()
