@@ -185,6 +185,82 @@ def served(n_jobs, rate, seed=0, steps_per_tick=25):
                        "cells), chiN 10-50, 500-5000 sweeps, 128 beads/Re^3, uint16"}
 
 
+def measure_system(workload, steps, warmup):
+    """Create the named system, autotune the sweep shape, run `warmup` steps,
+    then time `steps` steps (sweep + gridding + omega every sweep)."""
+    from new_arch_amd import scmf
+    from new_arch_amd import workloads as W
+    from new_arch_amd.abi import make_opts
+
+    spec = W.NAMED[workload]()
+    move = "trial" if workload == "small_diblock_melt" else "smart"
+    bits = 32 if workload == "largest_blend" else 16
+    opts = make_opts(seed=0, move=move, trial_delta=0.1, omega_every_step=True, density_bits=bits)
+    log(f"creating {workload}: {spec.n_beads} beads, {spec.n_chains} chains, grid {spec.n}")
+    t = time.perf_counter()
+    g = scmf.System(spec, opts)
+    g.grid_density()
+    g.update_omega()
+    g.sync()
+    log(f"created in {time.perf_counter() - t:.1f} s")
+    lanes, tuned_ms = g.autotune(2)
+    log(f"autotune: {lanes} lanes per chain, {tuned_ms:.3f} ms per sweep")
+    for i in range(warmup):
+        g.step(i + 1)
+    g.sync()
+    log(f"warmup {warmup} steps done")
+    g.enable_timing(True)
+    g.reset_timing()
+    g.reset_stats()
+    g.sync()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        g.step(warmup + i + 1)
+    g.sync()
+    t1 = time.perf_counter()
+    tm = g.timing()
+    st = g.stats()
+    g.close()
+    log(f"{workload}: timed {steps} steps in {t1 - t0:.3f} s; sweep {tm.sweep_ms:.3f} ms, "
+        f"grid {tm.grid_ms:.3f} ms, omega {tm.omega_ms:.3f} ms; acceptance "
+        f"{st.accepts / max(st.moves, 1):.3f}; grid fallback {st.grid_fallback}")
+
+    B, TC = spec.n_beads, spec.n_types * spec.n_cells
+    s = 2 if bits == 16 else 4
+    sweep_bytes = 24 * B + 4 * TC       # positions read + written, omega read once
+    grid_bytes = 12 * B + s * TC        # positions read, counts written
+    omega_bytes = s * TC + 4 * TC       # counts read, fp32 omega written
+    tiled = spec.n_beads >= (1 << 23)
+    sweep_kernel = {-1: "k_sweep_global", 1: "k_sweep_lds"}.get(lanes, f"k_sweep_sched<{lanes}>")
+
+    def roof(nbytes, ms):
+        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": nbytes, "ms": round(ms, 4)}
+
+    return {
+        "_spec": spec, "_opts": opts,
+        "workload": workload,
+        "bead_moves_per_s": B * steps / (t1 - t0),
+        "ms_per_step": (t1 - t0) * 1e3 / steps,
+        "steps": steps,
+        "dtype": "f32 coords / f64 energy+binning / " + ("u16" if bits == 16 else "u32") + " counts",
+        "config": {"workload": workload, "beads": B, "chains": spec.n_chains,
+                   "N": [p.length for p in spec.polymers], "grid": list(spec.n),
+                   "box_Re": list(spec.L), "xn": spec.xn, "move": move,
+                   "density_bits": bits, "omega": "every sweep", "parallelism": "1 GPU",
+                   "lanes_per_chain": lanes},
+        "roofline": dict(roof(sweep_bytes, tm.sweep_ms), kernel=sweep_kernel),
+        "passes": {"grid": dict(roof(grid_bytes, tm.grid_ms),
+                                kernel="k_zero_counts+k_grid_tiles+k_tiles_sum" if tiled
+                                else "k_zero_counts+k_grid"),
+                   "omega": dict(roof(omega_bytes, tm.omega_ms), kernel="k_omega")},
+        "acceptance": st.accepts / max(st.moves, 1),
+        "grid_fallback_beads": int(st.grid_fallback),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +270,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-largest", action="store_true")
     ap.add_argument("--served-jobs", type=int, default=200)
     ap.add_argument("--served-rate", type=float, default=6.0)
     args = ap.parse_args()
@@ -202,82 +279,31 @@ def main():
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
 
-    from new_arch_amd import scmf
-    from new_arch_amd import workloads as W
-    from new_arch_amd.abi import make_opts
-
-    spec = W.NAMED[args.workload]()
-    move = "trial" if args.workload == "small_diblock_melt" else "smart"
-    bits = 32 if args.workload == "largest_blend" else 16
-    opts = make_opts(seed=0, move=move, trial_delta=0.1, omega_every_step=True, density_bits=bits)
-    log(f"creating {args.workload}: {spec.n_beads} beads, {spec.n_chains} chains, "
-        f"grid {spec.n}")
-    t = time.perf_counter()
-    g = scmf.System(spec, opts)
-    g.grid_density()
-    g.update_omega()
-    g.sync()
-    log(f"created in {time.perf_counter() - t:.1f} s")
-    lanes, tuned_ms = g.autotune(2)
-    log(f"autotune: {lanes} lanes per chain, {tuned_ms:.3f} ms per sweep")
-    for i in range(args.warmup):
-        g.step(i + 1)
-    g.sync()
-    log(f"warmup {args.warmup} steps done")
-    g.enable_timing(True)
-    g.reset_timing()
-    g.reset_stats()
-    g.sync()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        g.step(args.warmup + i + 1)
-    g.sync()
-    t1 = time.perf_counter()
-    tm = g.timing()
-    st = g.stats()
-    log(f"timed {args.steps} steps in {t1 - t0:.3f} s; sweep {tm.sweep_ms:.3f} ms, "
-        f"grid {tm.grid_ms:.3f} ms, omega {tm.omega_ms:.3f} ms; acceptance "
-        f"{st.accepts / max(st.moves, 1):.3f}")
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = spec.n_beads * args.steps / (t1 - t0)
-
-    B, TC = spec.n_beads, spec.n_types * spec.n_cells
-    s = 2 if bits == 16 else 4
-    sweep_bytes = 24 * B + 4 * TC
-    grid_bytes = 12 * B + s * TC
-    omega_bytes = s * TC + 4 * TC
-
-    def roof(nbytes, ms):
-        gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": nbytes, "ms": round(ms, 4)}
-
+    m = measure_system(args.workload, args.steps, args.warmup)
+    spec, opts = m.pop("_spec"), m.pop("_opts")
     out = {
         "metric": "bead-moves/s/GPU",
-        "value": value,
+        "value": m["bead_moves_per_s"],
         "unit": "bead-moves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 coords / f64 energy+binning / u16 counts" if bits == 16 else
-                 "f32 coords / f64 energy+binning / u32 counts",
+        "dtype": m["dtype"],
         "data": "synthetic (seeded Gaussian-walk chains, uniform chain starts)",
-        "config": {"workload": args.workload, "beads": B, "chains": spec.n_chains,
-                   "N": [p.length for p in spec.polymers], "grid": list(spec.n),
-                   "box_Re": list(spec.L), "xn": spec.xn, "move": move,
-                   "density_bits": bits, "omega": "every sweep", "parallelism": "1 GPU",
-                   "lanes_per_chain": lanes},
-        "roofline": dict(roof(sweep_bytes, tm.sweep_ms), kernel="k_sweep"),
-        "passes": {"grid": dict(roof(grid_bytes, tm.grid_ms), kernel="k_zero+k_grid"),
-                   "omega": dict(roof(omega_bytes, tm.omega_ms), kernel="k_omega")},
-        "acceptance": st.accepts / max(st.moves, 1),
+        "config": m["config"],
+        "roofline": m["roofline"],
+        "passes": m["passes"],
+        "acceptance": m["acceptance"],
     }
-    g.close()
+    if not args.no_largest and args.workload != "largest_blend":
+        log("largest system")
+        lg = measure_system("largest_blend", steps=5, warmup=1)
+        lg.pop("_spec"), lg.pop("_opts")
+        out["largest"] = lg
     if not args.no_batched:
         log("batched vs plain")
         out["batched"] = batched_vs_plain(steps=50, warmup=5)

@@ -19,6 +19,14 @@
 // quotient |q_ref - t/d| <= 2u n (|m| + 2) (n ~ L/d).  The fast path's own
 // errors are no larger.  Requiring dist(q1, Z) > 2^-49 (|q1| + 1) and
 // dist(q2, Z) > 2^-48 n (|m| + 2), with |q1| < 2^30, leaves a margin above both.
+//
+// In-box path (fold).  For 0 <= r < L (1 - 2^-50) the reference's quotient
+// r/L rounds below 1, so k = 0, r - L*k = r exactly, and the cell is
+// trunc(fl(r/d)) mod n.  q = r * (1/d) differs from r/d by at most
+// 2^-52 q and fl(r/d) by at most 2^-53 q, with q < n + 1; when q lies
+// farther than 2^-50 (n + 2) from an integer both truncate to the same
+// cell (and q < n, so the mod is the identity).  Everything else takes
+// fold_fast.
 #pragma once
 #include <stdint.h>
 
@@ -32,6 +40,7 @@ namespace scmf {
 
 struct Axis {
   double L, d, invL, invd, guard2;  // guard2 = 2^-48 * n
+  double lhi, gin, gin1;            // L (1 - 2^-50); 2^-50 (n + 2); 1 - gin
   int n;
 };
 
@@ -78,6 +87,20 @@ SCMF_HD int fold_fast(double r, const Axis a) {
   return fold_exact(r, a.L, a.d, a.n);
 }
 
+// Same result as fold_exact for every input; the cheap case first.
+SCMF_HD int fold(double r, const Axis a) {
+  if (r >= 0.0 && r < a.lhi) {
+    const double q = r * a.invd;
+    const double c = __builtin_floor(q);
+    const double f = q - c;
+    if (f > a.gin && f < a.gin1) {
+      const int i = (int)c;
+      if ((unsigned)i < (unsigned)a.n) return i;
+    }
+  }
+  return fold_fast(r, a);
+}
+
 SCMF_HD Axis make_axis(double L, int n) {
   Axis a;
   a.L = L;
@@ -86,6 +109,9 @@ SCMF_HD Axis make_axis(double L, int n) {
   a.invL = 1.0 / L;
   a.invd = 1.0 / a.d;
   a.guard2 = 0x1p-48 * (double)n;
+  a.lhi = L * (1.0 - 0x1p-50);
+  a.gin = 0x1p-50 * ((double)n + 2.0);
+  a.gin1 = 1.0 - a.gin;
   return a;
 }
 

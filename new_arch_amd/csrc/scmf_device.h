@@ -115,6 +115,9 @@ __device__ __forceinline__ Axis axis(const KP& kp, int a) {
   r.invL = kp.ax[a].invL;
   r.invd = kp.ax[a].invd;
   r.guard2 = kp.ax[a].guard2;
+  r.lhi = kp.ax[a].lhi;
+  r.gin = kp.ax[a].gin;
+  r.gin1 = kp.ax[a].gin1;
   r.n = kp.ax[a].n;
   return r;
 }
@@ -122,9 +125,9 @@ __device__ __forceinline__ Axis axis(const KP& kp, int a) {
 template <class KP>
 __device__ __forceinline__ uint32_t bin_cell(const KP& kp, double rx, double ry, double rz,
                                              uint32_t type) {
-  const int xi = fold_fast(rx, axis(kp, 0));
-  const int yi = fold_fast(ry, axis(kp, 1));
-  const int zi = fold_fast(rz, axis(kp, 2));
+  const int xi = fold(rx, axis(kp, 0));
+  const int yi = fold(ry, axis(kp, 1));
+  const int zi = fold(rz, axis(kp, 2));
   return type * kp.n_cells + (uint32_t)(xi + yi * kp.nx + zi * kp.nx * kp.ny);
 }
 

@@ -638,6 +638,10 @@ struct Timed {
   hipEvent_t a, b;
 };
 
+// Position arrays hold a multiple of four beads (the gridding reads them
+// four at a time).
+int64_t padded(int64_t beads) { return (beads + 3) & ~(int64_t)3; }
+
 unsigned grid_for(int64_t n) {
   int64_t g = (n + BLOCK - 1) / BLOCK;
   if (g < 1) g = 1;
@@ -676,7 +680,7 @@ struct scmf_sys {
   float* x2 = nullptr;
   float* y2 = nullptr;
   float* z2 = nullptr;
-  uint32_t* scratch = nullptr;
+  uint16_t* scratch = nullptr;  // window images of the tiled gridding
   unsigned long long* d_fallback = nullptr;
   TileGeo geo{};
   size_t tile_lds = 0;  // LDS of k_grid_tiles; 0 = tiled gridding not usable
@@ -962,48 +966,96 @@ int launch_sweep(scmf_sys* s) {
 // included) at least h cells long, so a cell is covered only by the windows of
 // its own tile and the two next to it.  The budget is the one maximising
 // interior/window volume times resident workgroups per CU.
+// Tile geometry of the tiled gridding.  The halo covers the reach of a
+// typical chain from its middle bead (0.75 Re scaled by sqrt((N-1)/(Nref-1))
+// for the bead-weighted median length N); beads beyond it are counted with
+// global atomics.  Among the tilings whose LDS window fits, the one with the
+// least window area per interior cell wins (that ratio sets the scratch
+// traffic), with a penalty for one workgroup per CU and for too few tiles to
+// fill the GPU.
 void choose_tiles(scmf_sys* s) {
   const KParams& kp = s->kp;
   s->tile_lds = 0;
-  const double dmin = std::min(kp.dx, std::min(kp.dy, kp.dz));
-  const double reach = 0.75 * sqrt((double)std::max(s->nmax - 1, 1) / (double)(s->desc.nref - 1));
-  int h = (int)ceil(reach / dmin);
-  h = std::max(2, std::min(h, 16));
-  const int n[3] = {kp.nx, kp.ny, kp.nz};
-  double best = 0;
-  TileGeo bg{};
-  size_t bbytes = 0;
-  for (size_t budget : {(size_t)64 << 10, (size_t)96 << 10, (size_t)144 << 10}) {
-    int Wc = 1;  // cube edge the budget allows
-    while ((size_t)(Wc + 1) * (Wc + 1) * (Wc + 1) * s->T * 2 <= budget) ++Wc;
-    TileGeo g{};
-    g.h = h;
-    bool ok = true;
-    for (int a = 0; a < 3 && ok; ++a) {
-      int ts = std::min(Wc, n[a]) - 2 * h;
-      while (ts >= h && n[a] % ts != 0 && n[a] % ts < h) --ts;
-      if (ts < h || ts < 2) ok = false;
-      g.ts[a] = ts;
-      g.W[a] = ts + 2 * h;
-    }
-    if (!ok) continue;
-    const double frac = (double)g.ts[0] * g.ts[1] * g.ts[2] / ((double)g.W[0] * g.W[1] * g.W[2]);
-    const size_t bytes = (((size_t)g.W[0] * g.W[1] * g.W[2] * s->T + 1) / 2) * 4;
-    const int per_cu = (int)std::max<size_t>(1, ((size_t)160 << 10) / bytes);
-    const double score = frac * std::min(per_cu, 2);
-    if (score > best) {
-      best = score;
-      bbytes = bytes;
-      bg = g;
+  std::vector<std::pair<int, int64_t>> lens;
+  int64_t total = 0;
+  for (int p = 0; p < kp.P; ++p) {
+    const int64_t b = kp.bead_base[p + 1] - kp.bead_base[p];
+    lens.push_back({kp.len[p], b});
+    total += b;
+  }
+  std::sort(lens.begin(), lens.end());
+  int nmed = lens.back().first;
+  int64_t acc = 0;
+  for (auto& l : lens) {
+    acc += l.second;
+    if (2 * acc >= total) {
+      nmed = l.first;
+      break;
     }
   }
+  const double reach = 0.75 * sqrt((double)std::max(nmed - 1, 1) / (double)std::max(s->desc.nref - 1, 1));
+  const int n[3] = {kp.nx, kp.ny, kp.nz};
+  const double d[3] = {kp.dx, kp.dy, kp.dz};
+  int hh[3];
+  for (int a = 0; a < 3; ++a) hh[a] = std::max(1, std::min(16, (int)ceil(reach / d[a])));
+  const size_t lds_cap = ((size_t)160 << 10) - MONO_LDS - 512;
+  // candidate tile counts per axis: one tile, or pieces no shorter than the
+  // halo and no longer than 64 cells
+  std::vector<int> cand[3];
+  for (int a = 0; a < 3; ++a) {
+    cand[a].push_back(1);
+    for (int nt = 2; nt <= n[a]; ++nt) {
+      if (n[a] / nt < hh[a]) break;
+      if ((n[a] + nt - 1) / nt <= 64) cand[a].push_back(nt);
+    }
+  }
+  double best = 0;
+  TileGeo bg{};
+  for (int n0 : cand[0])
+    for (int n1 : cand[1])
+      for (int n2 : cand[2]) {
+        const int nts[3] = {n0, n1, n2};
+        TileGeo g{};
+        int64_t wc = 1, ic = 1;
+        for (int a = 0; a < 3; ++a) {
+          g.nt[a] = nts[a];
+          g.q[a] = n[a] / nts[a];
+          g.rem[a] = n[a] % nts[a];
+          g.h[a] = nts[a] == 1 ? 0 : hh[a];
+          g.E[a] = g.q[a] + (g.rem[a] ? 1 : 0);
+          g.W[a] = g.E[a] + 2 * g.h[a];
+          wc *= g.W[a];
+          ic *= g.E[a];
+        }
+        const size_t bytes = (size_t)((wc * s->T + 1) / 2) * 4;
+        if (bytes > lds_cap) continue;
+        const int64_t ntiles = (int64_t)n0 * n1 * n2;
+        const int per_cu = (int)std::min<size_t>(2, ((size_t)160 << 10) / (bytes + MONO_LDS + 512));
+        double cost = (double)wc / (double)ic;
+        if (per_cu < 2) cost *= 1.15;
+        if (ntiles < 1024) cost *= 1024.0 / (double)ntiles;
+        const double score = 1.0 / cost;
+        if (score > best) {
+          best = score;
+          bg = g;
+          bg.ntiles = (int)ntiles;
+          bg.win_cells = (int)wc;
+          bg.win_words = (int)((wc * s->T + 1) / 2);
+        }
+      }
   if (best <= 0) return;
-  for (int a = 0; a < 3; ++a) bg.nt[a] = (n[a] + bg.ts[a] - 1) / bg.ts[a];
-  bg.ntiles = bg.nt[0] * bg.nt[1] * bg.nt[2];
-  bg.win_cells = bg.W[0] * bg.W[1] * bg.W[2];
-  bg.win_words = (int)(((int64_t)bg.win_cells * s->T + 1) / 2);
+  // region offsets of a window image: each region sized for the largest tile
+  int off = 0;
+  for (int r = 0; r < 27; ++r) {
+    const int rr[3] = {r % 3, (r / 3) % 3, r / 9};
+    int64_t e = s->T;
+    for (int a = 0; a < 3; ++a) e *= rr[a] == 1 ? bg.E[a] : bg.h[a];
+    bg.reg_off[r] = off;
+    off += (int)((e + 1) & ~(int64_t)1);
+  }
+  bg.slot_elems = (off + 63) & ~63;
   s->geo = bg;
-  s->tile_lds = bbytes;
+  s->tile_lds = (size_t)bg.win_words * 4;
 }
 
 int alloc_sort_buffers(scmf_sys* s) {
@@ -1025,11 +1077,12 @@ int alloc_sort_buffers(scmf_sys* s) {
       (rc = A((void**)&s->tcount, sizeof(uint32_t) * nk)) ||
       (rc = A((void**)&s->tfill, sizeof(uint32_t) * nk)) ||
       (rc = A((void**)&s->tstart, sizeof(uint32_t) * (nk + 1))) ||
-      (rc = A((void**)&s->x2, sizeof(float) * B)) || (rc = A((void**)&s->y2, sizeof(float) * B)) ||
-      (rc = A((void**)&s->z2, sizeof(float) * B)) ||
-      (rc = A((void**)&s->scratch, sizeof(uint32_t) * (size_t)s->geo.ntiles * s->geo.win_words)))
+      (rc = A((void**)&s->x2, sizeof(float) * padded(B))) ||
+      (rc = A((void**)&s->y2, sizeof(float) * padded(B))) ||
+      (rc = A((void**)&s->z2, sizeof(float) * padded(B))) ||
+      (rc = A((void**)&s->scratch, sizeof(uint16_t) * (size_t)s->geo.ntiles * s->geo.slot_elems)))
     return rc;
-  if (s->tile_lds > 65536 &&
+  if (s->tile_lds + MONO_LDS + 128 > 65536 &&
       (hipFuncSetAttribute((const void*)k_grid_tiles<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
                            (int)s->tile_lds) != hipSuccess ||
        hipFuncSetAttribute((const void*)k_grid_tiles<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1078,15 +1131,15 @@ int sort_chains(scmf_sys* s) {
 int launch_tiled_grid(scmf_sys* s, hipStream_t st, const KParams* Ps, int sys) {
   const dim3 tiles((unsigned)s->geo.ntiles);
   if (s->opts.density_bits == 32) {
-    hipLaunchKernelGGL(k_grid_tiles<32>, tiles, dim3(1024), s->tile_lds, st, Ps, sys, s->geo,
-                       (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
-    hipLaunchKernelGGL(k_tiles_sum<32>, dim3(grid_strided(s->count_words)), dim3(256), 0, st, Ps,
-                       sys, s->geo, (const uint32_t*)s->scratch);
+    hipLaunchKernelGGL(k_grid_tiles<32>, tiles, dim3(GRID_THREADS), s->tile_lds, st, Ps, sys,
+                       s->geo, (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
+    hipLaunchKernelGGL(k_tiles_sum<32>, tiles, dim3(256), 0, st, Ps, sys, s->geo,
+                       (const uint16_t*)s->scratch);
   } else {
-    hipLaunchKernelGGL(k_grid_tiles<16>, tiles, dim3(1024), s->tile_lds, st, Ps, sys, s->geo,
-                       (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
-    hipLaunchKernelGGL(k_tiles_sum<16>, dim3(grid_strided(s->count_words)), dim3(256), 0, st, Ps,
-                       sys, s->geo, (const uint32_t*)s->scratch);
+    hipLaunchKernelGGL(k_grid_tiles<16>, tiles, dim3(GRID_THREADS), s->tile_lds, st, Ps, sys,
+                       s->geo, (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
+    hipLaunchKernelGGL(k_tiles_sum<16>, tiles, dim3(256), 0, st, Ps, sys, s->geo,
+                       (const uint16_t*)s->scratch);
   }
   HIP_TRY(hipGetLastError());
   return SCMF_OK;
@@ -1163,8 +1216,9 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
     }
     return SCMF_OK;
   };
-  if ((rc = alloc((void**)&s->x, sizeof(float) * B)) || (rc = alloc((void**)&s->y, sizeof(float) * B)) ||
-      (rc = alloc((void**)&s->z, sizeof(float) * B)) ||
+  if ((rc = alloc((void**)&s->x, sizeof(float) * padded(B))) ||
+      (rc = alloc((void**)&s->y, sizeof(float) * padded(B))) ||
+      (rc = alloc((void**)&s->z, sizeof(float) * padded(B))) ||
       (rc = alloc((void**)&s->counts, sizeof(uint32_t) * s->count_words)) ||
       (rc = alloc((void**)&s->omega, sizeof(float) * TC)) ||
       (rc = alloc((void**)&s->d_accepts, sizeof(unsigned long long) * ACC_SLOTS)) ||

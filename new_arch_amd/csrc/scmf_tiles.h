@@ -7,29 +7,57 @@
 //      within each architecture block (so bead offsets stay a formula of the
 //      slot index and the per-bead monomer types never move);
 //   2. one workgroup per tile bins its chains' beads into an LDS window
-//      (tile + halo, uint16 counters packed in pairs) and stores the window
-//      to a scratch buffer with plain coalesced stores;
-//   3. one lane per pair of cells sums the windows that cover it.
+//      (the tile plus a halo of h cells per side, uint16 counters packed in
+//      pairs), four beads per lane from 16-byte loads, and stores the window
+//      as 27 contiguous regions (low halo, interior, high halo per axis);
+//   3. one workgroup per tile sums, for each of its cells, the regions of
+//      the (up to 27) windows that cover it — every region is read once,
+//      contiguously.
 // A bead outside its tile's window (a chain that drifted since the last
 // ordering) is added with a global atomic instead, so results never depend
 // on how well ordered the chains are; only speed does.
+//
+// Tiles split each axis into nt pieces of q or q+1 cells (the first `rem`
+// pieces get q+1); a piece is never shorter than the halo, so every halo
+// region of a window lies inside one neighbouring tile.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "scmf_device.h"
 
 namespace scmf {
 
 struct TileGeo {
-  int ts[3];        // tile edge in cells
-  int nt[3];        // tiles per axis
-  int h;            // halo in cells
-  int W[3];         // window edge = ts + 2h (<= grid edge)
-  int ntiles;       // nt0 nt1 nt2
-  int win_cells;    // W0 W1 W2
-  int win_words;    // T win_cells / 2, rounded up (uint16 pairs)
+  int nt[3];          // tiles per axis
+  int q[3], rem[3];   // tile i along an axis has q + (i < rem) cells
+  int h[3];           // halo in cells (0 on an axis with a single tile)
+  int E[3];           // largest tile edge, q + (rem > 0)
+  int W[3];           // LDS window box edge, E + 2h
+  int ntiles;         // nt0 nt1 nt2
+  int win_cells;      // W0 W1 W2
+  int win_words;      // T win_cells / 2, rounded up (uint16 pairs)
+  int slot_elems;     // uint16 elements of one window image in scratch
+  int reg_off[27];    // element offset of region rx + 3 ry + 9 rz in an image
 };
+
+// Regions along an axis: 0 = low halo, 1 = interior, 2 = high halo.
+__host__ __device__ __forceinline__ int tile_len(int q, int rem, int i) { return q + (i < rem ? 1 : 0); }
+__host__ __device__ __forceinline__ int tile_first(int q, int rem, int i) {
+  return i * q + (i < rem ? i : rem);
+}
+__host__ __device__ __forceinline__ int tile_index(int q, int rem, int c) {
+  const int big = rem * (q + 1);
+  return c < big ? c / (q + 1) : rem + (c - big) / q;
+}
+
+// floor(a / b) for 0 <= a, 0 < b, a + b < 2^21: (a + 1/2)/b lies at least
+// 1/(2b) from an integer, more than the error of the fp32 product.
+__device__ __forceinline__ int small_div(int a, float inv_b) {
+  return (int)(((float)a + 0.5f) * inv_b);
+}
 
 // ---------------------------------------------------------------- ordering
 // Key of every slot: architecture block, then tile of the chain's middle bead.
@@ -43,10 +71,12 @@ __global__ void __launch_bounds__(256) k_tile_keys(KParams kp, TileGeo g,
   if (s >= kp.n_chains) return;
   const int p = arch_of_chain(kp, s);
   const int64_t a = kp.bead_base[p] + (s - kp.chain_base[p]) * kp.len[p] + kp.len[p] / 2;
-  const int xi = fold_fast((double)x[a], axis(kp, 0));
-  const int yi = fold_fast((double)y[a], axis(kp, 1));
-  const int zi = fold_fast((double)z[a], axis(kp, 2));
-  const uint32_t t = (uint32_t)(xi / g.ts[0] + g.nt[0] * (yi / g.ts[1] + g.nt[1] * (zi / g.ts[2])));
+  const int xi = fold((double)x[a], axis(kp, 0));
+  const int yi = fold((double)y[a], axis(kp, 1));
+  const int zi = fold((double)z[a], axis(kp, 2));
+  const uint32_t t = (uint32_t)(tile_index(g.q[0], g.rem[0], xi) +
+                                g.nt[0] * (tile_index(g.q[1], g.rem[1], yi) +
+                                           g.nt[1] * tile_index(g.q[2], g.rem[2], zi)));
   const uint32_t key = (uint32_t)p * (uint32_t)g.ntiles + t;
   keys[s] = key;
   atomicAdd(&count[key], 1u);
@@ -142,17 +172,69 @@ __device__ __forceinline__ int wrap_rel(int c, int origin, int n) {
   return d;
 }
 
-// One workgroup per tile: bin the beads of the tile's chains into the LDS
-// window, store the window.  Beads outside the window go straight to the
-// final counts with a global atomic.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int MONO_LDS = 4096;     // architectures up to this length read types from LDS
+constexpr int GRID_THREADS = 1024;
+// Beads binned between two normalisations of the window: each uint16 half
+// then grows by at most this much from below 2^15, so it cannot wrap.
+constexpr int NORM_BEADS = 7 * 4 * GRID_THREADS;
+
+// Counts `v` more beads of global cell `cell` with a global atomic.
 template <int BITS>
-__global__ void __launch_bounds__(1024) k_grid_tiles(const KParams* __restrict__ Ps, int sys,
-                                                     TileGeo g,
-                                                     const uint32_t* __restrict__ tile_start,
-                                                     uint32_t* __restrict__ scratch,
-                                                     unsigned long long* __restrict__ fallback) {
+__device__ __forceinline__ void add_global(SCMF_GLOBAL uint32_t* counts, uint32_t* overflow,
+                                           uint32_t cell, uint32_t v) {
+  if (BITS == 32) {
+    atomicAdd((uint32_t*)&counts[cell], v);
+  } else {
+    const uint32_t sh = (cell & 1u) * 16u;
+    const uint32_t old = atomicAdd((uint32_t*)&counts[cell >> 1], v << sh);
+    if (((old >> sh) & 0xFFFFu) + v > 0xFFFFu) atomicOr(overflow, 1u);
+  }
+}
+
+// Move every uint16 half of the window that has reached 2^15 to the global
+// counts and clear it (only tiles with more than 65535 beads need this).
+template <int BITS>
+__device__ void normalise_window(uint32_t* win, const TileGeo& g, KPC& kp, int ox, int oy, int oz,
+                                 SCMF_GLOBAL uint32_t* counts, uint32_t* overflow) {
+  for (int w = threadIdx.x; w < g.win_words; w += GRID_THREADS) {
+    const uint32_t v = win[w];
+    if (!(v & 0x80008000u)) continue;
+    uint32_t keep = v;
+    for (int half = 0; half < 2; ++half) {
+      const uint32_t val = (v >> (16 * half)) & 0xFFFFu;
+      if (val < 0x8000u) continue;
+      keep &= ~(0xFFFFu << (16 * half));
+      const int wc = 2 * w + half;
+      const int type = wc / g.win_cells;
+      const int r = wc - type * g.win_cells;
+      const int wx = r % g.W[0], wy = (r / g.W[0]) % g.W[1], wz = r / (g.W[0] * g.W[1]);
+      const int cx = wrap_rel(ox + wx, 0, kp.nx), cy = wrap_rel(oy + wy, 0, kp.ny),
+                cz = wrap_rel(oz + wz, 0, kp.nz);
+      add_global<BITS>(counts, overflow,
+                       (uint32_t)type * kp.n_cells + (uint32_t)(cx + kp.nx * (cy + kp.ny * cz)), val);
+    }
+    win[w] = keep;
+  }
+}
+
+// One workgroup per tile: bin the beads of the tile's chains into the LDS
+// window, then store the window as 27 regions.  Beads outside the window go
+// straight to the final counts with a global atomic.  Positions are read
+// four beads per lane (16-byte loads; the arrays are padded to a multiple of
+// four beads), the next four in flight while the current ones are binned.
+template <int BITS>
+__global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __restrict__ Ps, int sys,
+                                                             TileGeo g,
+                                                             const uint32_t* __restrict__ tile_start,
+                                                             uint16_t* __restrict__ scratch,
+                                                             unsigned long long* __restrict__ fallback) {
   extern __shared__ uint32_t win[];
+  __shared__ uint8_t mono_s[MONO_LDS];
+  __shared__ int reg_s[27];
   KPC& kp = param(Ps, (uint32_t)sys);
+  if (threadIdx.x < 27) reg_s[threadIdx.x] = g.reg_off[threadIdx.x];
   const SCMF_GLOBAL float* __restrict__ x = gptr((const float*)kp.x);
   const SCMF_GLOBAL float* __restrict__ y = gptr((const float*)kp.y);
   const SCMF_GLOBAL float* __restrict__ z = gptr((const float*)kp.z);
@@ -160,129 +242,216 @@ __global__ void __launch_bounds__(1024) k_grid_tiles(const KParams* __restrict__
   SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
   uint32_t* __restrict__ overflow = (uint32_t*)kp.overflow;
   const int t = blockIdx.x;
-  for (int w = threadIdx.x; w < g.win_words; w += 1024) win[w] = 0u;
-  __syncthreads();
   const int tx = t % g.nt[0], ty = (t / g.nt[0]) % g.nt[1], tz = t / (g.nt[0] * g.nt[1]);
-  const int ox = tx * g.ts[0] - g.h, oy = ty * g.ts[1] - g.h, oz = tz * g.ts[2] - g.h;
+  const int sx = tile_len(g.q[0], g.rem[0], tx), sy = tile_len(g.q[1], g.rem[1], ty),
+            sz = tile_len(g.q[2], g.rem[2], tz);
+  const int ox = tile_first(g.q[0], g.rem[0], tx) - g.h[0],
+            oy = tile_first(g.q[1], g.rem[1], ty) - g.h[1],
+            oz = tile_first(g.q[2], g.rem[2], tz) - g.h[2];
+  const int wxm = sx + 2 * g.h[0], wym = sy + 2 * g.h[1], wzm = sz + 2 * g.h[2];
+  for (int w = threadIdx.x; w < g.win_words; w += GRID_THREADS) win[w] = 0u;
+  int64_t pop = 0;
+  for (int p = 0; p < kp.P; ++p)
+    pop += (int64_t)(tile_start[p * g.ntiles + t + 1] - tile_start[p * g.ntiles + t]) * kp.len[p];
+  const bool checked = pop > 65535;
+  const Axis ax0 = axis(kp, 0), ax1 = axis(kp, 1), ax2 = axis(kp, 2);
+  uint32_t missed = 0;
+  int since = 0;
   for (int p = 0; p < kp.P; ++p) {
     const int64_t s0 = tile_start[p * g.ntiles + t], s1 = tile_start[p * g.ntiles + t + 1];
     if (s1 <= s0) continue;
     const int len = kp.len[p];
-    const int64_t b0 = kp.bead_base[p] + (s0 - kp.chain_base[p]) * len;
-    const int64_t b1 = kp.bead_base[p] + (s1 - kp.chain_base[p]) * len;
-    for (int64_t gb = b0 + threadIdx.x; gb < b1; gb += 1024) {
-      const uint32_t ty_ = btype[gb];
-      const int xi = fold_fast((double)x[gb], axis(kp, 0));
-      const int yi = fold_fast((double)y[gb], axis(kp, 1));
-      const int zi = fold_fast((double)z[gb], axis(kp, 2));
-      const int wx = wrap_rel(xi, ox, kp.nx), wy = wrap_rel(yi, oy, kp.ny),
-                wz = wrap_rel(zi, oz, kp.nz);
-      bool done = false;
-      if (wx < g.W[0] && wy < g.W[1] && wz < g.W[2]) {
-        const uint32_t wc = ty_ * (uint32_t)g.win_cells + (uint32_t)(wx + g.W[0] * (wy + g.W[1] * wz));
-        const uint32_t sh = (wc & 1u) * 16u;
-        const uint32_t old = atomicAdd(&win[wc >> 1], 1u << sh);
-        if (((old >> sh) & 0xFFFFu) == 0xFFFFu) {
-          atomicSub(&win[wc >> 1], 1u << sh);  // this half is full: undo, count globally
-        } else {
-          done = true;
+    const int64_t bb = kp.bead_base[p];
+    const int64_t b0 = bb + (s0 - kp.chain_base[p]) * len;
+    const int64_t b1 = bb + (s1 - kp.chain_base[p]) * len;
+    const bool tab = len <= MONO_LDS;
+    __syncthreads();
+    if (tab)
+      for (int i = threadIdx.x; i < len; i += GRID_THREADS)
+        mono_s[i] = (uint8_t)kp.monotype[kp.mono_off[p] + i];
+    __syncthreads();
+    const double inv_len = kp.inv_len[p];
+    // The type source is fixed per architecture; two copies of the loop keep
+    // a global type load from sharing registers with the LDS one (which
+    // would make every LDS read wait for the prefetched positions).
+    auto bin_range = [&](auto tab_c) {
+      constexpr bool TAB = decltype(tab_c)::value;
+      // Loads are issued unconditionally (lanes past the range re-read the
+      // last group of the range), so the wait before binning counts only
+      // the current group and the next group stays in flight.
+      const int64_t last = (b1 - 1) & ~(int64_t)3;
+      int64_t gq = (b0 & ~(int64_t)3) + 4 * (int64_t)threadIdx.x;
+      const int64_t a0 = gq < b1 ? gq : last;
+      f32x4 X = *(const SCMF_GLOBAL f32x4*)(x + a0);
+      f32x4 Y = *(const SCMF_GLOBAL f32x4*)(y + a0);
+      f32x4 Z = *(const SCMF_GLOBAL f32x4*)(z + a0);
+      for (int64_t base = b0 & ~(int64_t)3; base < b1; base += 4 * GRID_THREADS) {
+        const int64_t gn = gq + 4 * GRID_THREADS;
+        const int64_t an = gn < b1 ? gn : last;
+        const f32x4 Xn = *(const SCMF_GLOBAL f32x4*)(x + an);
+        const f32x4 Yn = *(const SCMF_GLOBAL f32x4*)(y + an);
+        const f32x4 Zn = *(const SCMF_GLOBAL f32x4*)(z + an);
+        if (gq < b1) {
+          // monomer index of the first bead of the four (floor mod len)
+          const int64_t rel = gq - bb;
+          const int64_t qd = (int64_t)__builtin_floor((double)rel * inv_len);
+          int m = (int)(rel - qd * len);
+          if (m < 0) m += len;
+          if (m >= len) m -= len;
+          const float px[4] = {X.x, X.y, X.z, X.w}, py[4] = {Y.x, Y.y, Y.z, Y.w},
+                      pz[4] = {Z.x, Z.y, Z.z, Z.w};
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int64_t gb = gq + e;
+            if (gb >= b0 && gb < b1) {
+              const uint32_t ty_ = TAB ? (uint32_t)mono_s[m] : (uint32_t)btype[gb];
+              const int xi = fold((double)px[e], ax0);
+              const int yi = fold((double)py[e], ax1);
+              const int zi = fold((double)pz[e], ax2);
+              const int wx = wrap_rel(xi, ox, kp.nx), wy = wrap_rel(yi, oy, kp.ny),
+                        wz = wrap_rel(zi, oz, kp.nz);
+              if (wx < wxm && wy < wym && wz < wzm) {
+                const uint32_t wc =
+                    ty_ * (uint32_t)g.win_cells + (uint32_t)(wx + g.W[0] * (wy + g.W[1] * wz));
+                atomicAdd(&win[wc >> 1], 1u << ((wc & 1u) * 16u));
+              } else {
+                ++missed;
+                add_global<BITS>(counts, overflow,
+                                 ty_ * kp.n_cells + (uint32_t)(xi + kp.nx * (yi + kp.ny * zi)), 1u);
+              }
+            }
+            if (++m == len) m = 0;
+          }
+        }
+        X = Xn;
+        Y = Yn;
+        Z = Zn;
+        gq = gn;
+        if (checked && (since += 4 * GRID_THREADS) >= NORM_BEADS) {
+          since = 0;
+          __syncthreads();
+          normalise_window<BITS>(win, g, kp, ox, oy, oz, counts, overflow);
+          __syncthreads();
         }
       }
-      const uint64_t missed = __ballot(!done);
-      if (missed && (threadIdx.x & 63) == (uint32_t)(__ffsll((unsigned long long)missed) - 1))
-        atomicAdd(fallback, (unsigned long long)__popcll(missed));
-      if (!done) {
-        const uint32_t cell = ty_ * kp.n_cells + (uint32_t)(xi + kp.nx * (yi + kp.ny * zi));
-        if (BITS == 32) {
-          atomicAdd((uint32_t*)&counts[cell], 1u);
-        } else {
-          const uint32_t sh = (cell & 1u) * 16u;
-          const uint32_t old = atomicAdd((uint32_t*)&counts[cell >> 1], 1u << sh);
-          if (((old >> sh) & 0xFFFFu) == 0xFFFFu) atomicOr(overflow, 1u);
-        }
-      }
-    }
+    };
+    if (tab)
+      bin_range(std::true_type{});
+    else
+      bin_range(std::false_type{});
   }
+  // beads counted globally: one atomic per wave
+  for (int off = 32; off > 0; off >>= 1) missed += __shfl_xor(missed, off);
+  if ((threadIdx.x & 63) == 0 && missed) atomicAdd(fallback, (unsigned long long)missed);
   __syncthreads();
-  uint32_t* dst = scratch + (int64_t)t * g.win_words;
-  for (int w = threadIdx.x; w < g.win_words; w += 1024) dst[w] = win[w];
-}
-
-// Windows of the tiles covering coordinate c along one axis: up to three
-// (tile, offset-in-window) pairs.
-__device__ __forceinline__ int covering(int c, int ts, int nt, int h, int W, int n, int* tile,
-                                        int* off) {
-  const int home = c / ts;
-  int k = 0;
-  for (int d = -1; d <= 1; ++d) {
-    int tt = home + d;
-    if (tt < 0) tt += nt;
-    if (tt >= nt) tt -= nt;
-    const int o = wrap_rel(c, tt * ts - h, n);
-    if (o < W) {
-      bool dup = false;
-      for (int j = 0; j < k; ++j) dup |= tile[j] == tt;
-      if (!dup) {
-        tile[k] = tt;
-        off[k] = o;
-        ++k;
-      }
+  // store the window as 27 regions
+  uint16_t* __restrict__ img = scratch + (int64_t)t * g.slot_elems;
+  const int sz3[3] = {sx, sy, sz};
+  for (int r = 0; r < 27; ++r) {
+    const int rx = r % 3, ry = (r / 3) % 3, rz = r / 9;
+    const int ex = rx == 1 ? sz3[0] : g.h[0], ey = ry == 1 ? sz3[1] : g.h[1],
+              ez = rz == 1 ? sz3[2] : g.h[2];
+    const int fx = rx == 0 ? 0 : (rx == 1 ? g.h[0] : g.h[0] + sx);
+    const int fy = ry == 0 ? 0 : (ry == 1 ? g.h[1] : g.h[1] + sy);
+    const int fz = rz == 0 ? 0 : (rz == 1 ? g.h[2] : g.h[2] + sz);
+    const int nrow = ex * ey * ez;
+    const int n_r = kp.T * nrow;
+    if (n_r == 0) continue;
+    const float iex = 1.0f / (float)ex, iey = 1.0f / (float)ey, irow = 1.0f / (float)nrow;
+    uint16_t* __restrict__ dst = img + reg_s[r];
+    for (int e = threadIdx.x; e < n_r; e += GRID_THREADS) {
+      const int type = small_div(e, irow);
+      const int l = e - type * nrow;
+      const int lyz = small_div(l, iex);
+      const int lx = l - lyz * ex;
+      const int lz = small_div(lyz, iey);
+      const int ly = lyz - lz * ey;
+      const uint32_t wc = (uint32_t)type * (uint32_t)g.win_cells +
+                          (uint32_t)((fx + lx) + g.W[0] * ((fy + ly) + g.W[1] * (fz + lz)));
+      dst[e] = (uint16_t)(win[wc >> 1] >> ((wc & 1u) * 16u));
     }
   }
-  return k;
 }
 
-__device__ __forceinline__ uint32_t window_count(const uint32_t* scratch, const TileGeo& g, int t,
-                                                 uint32_t type, int wx, int wy, int wz) {
-  const uint32_t wc = type * (uint32_t)g.win_cells + (uint32_t)(wx + g.W[0] * (wy + g.W[1] * wz));
-  return (scratch[(int64_t)t * g.win_words + (wc >> 1)] >> ((wc & 1u) * 16u)) & 0xFFFFu;
-}
-
-// Sum the windows into the final counts (which already hold the fallback
-// atomics).  One lane per 32-bit word of counts: one cell (uint32) or a pair
-// of cells (uint16), so no two lanes write the same word.
+// One workgroup per tile: every cell of the tile gets its window's interior
+// value, the halo values of the neighbouring windows that reach it, and the
+// globally counted beads already in the counts.
 template <int BITS>
 __global__ void __launch_bounds__(256) k_tiles_sum(const KParams* __restrict__ Ps, int sys,
                                                    TileGeo g,
-                                                   const uint32_t* __restrict__ scratch) {
+                                                   const uint16_t* __restrict__ scratch) {
   KPC& kp = param(Ps, (uint32_t)sys);
   SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
+  SCMF_GLOBAL uint16_t* __restrict__ counts16 = gptr((uint16_t*)kp.counts);
   uint32_t* __restrict__ overflow = (uint32_t*)kp.overflow;
-  const int64_t n_words = kp.count_words;
-  const int64_t stride = (int64_t)gridDim.x * 256;
-  for (int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x; w < n_words; w += stride) {
-    uint32_t out = 0;
-    const int per_word = BITS == 32 ? 1 : 2;
-    for (int h2 = 0; h2 < per_word; ++h2) {
-      const int64_t idx = w * per_word + h2;
-      if (idx >= (int64_t)kp.T * kp.n_cells) break;
-      const uint32_t type = (uint32_t)(idx / kp.n_cells);
-      const uint32_t c = (uint32_t)(idx - (int64_t)type * kp.n_cells);
-      const uint32_t row = c / (uint32_t)kp.nx;
-      const int xi = (int)(c - row * (uint32_t)kp.nx);
-      const int yi = (int)(row % (uint32_t)kp.ny), zi = (int)(row / (uint32_t)kp.ny);
-      int txs[3], oxs[3], tys[3], oys[3], tzs[3], ozs[3];
-      const int kx = covering(xi, g.ts[0], g.nt[0], g.h, g.W[0], kp.nx, txs, oxs);
-      const int ky = covering(yi, g.ts[1], g.nt[1], g.h, g.W[1], kp.ny, tys, oys);
-      const int kz = covering(zi, g.ts[2], g.nt[2], g.h, g.W[2], kp.nz, tzs, ozs);
-      uint32_t sum = BITS == 32 ? counts[w] : (counts[w] >> (16 * h2)) & 0xFFFFu;
-      for (int a = 0; a < kz; ++a)
-        for (int b = 0; b < ky; ++b)
-          for (int d = 0; d < kx; ++d) {
-            const int t = txs[d] + g.nt[0] * (tys[b] + g.nt[1] * tzs[a]);
-            sum += window_count(scratch, g, t, type, oxs[d], oys[b], ozs[a]);
-          }
-      if (BITS == 32) {
-        out = sum;
-      } else {
-        if (sum > 0xFFFFu) {
-          atomicOr(overflow, 1u);
-          sum = 0xFFFFu;
+  __shared__ int reg_s[27];
+  if (threadIdx.x < 27) reg_s[threadIdx.x] = g.reg_off[threadIdx.x];
+  __syncthreads();
+  const int t = blockIdx.x;
+  const int v0 = t % g.nt[0], v1 = (t / g.nt[0]) % g.nt[1], v2 = t / (g.nt[0] * g.nt[1]);
+  const int s0 = tile_len(g.q[0], g.rem[0], v0), s1 = tile_len(g.q[1], g.rem[1], v1),
+            s2 = tile_len(g.q[2], g.rem[2], v2);
+  const int f0 = tile_first(g.q[0], g.rem[0], v0), f1 = tile_first(g.q[1], g.rem[1], v1),
+            f2 = tile_first(g.q[2], g.rem[2], v2);
+  // neighbouring windows along each axis (periodic)
+  const int d0 = v0 == 0 ? g.nt[0] - 1 : v0 - 1, u0 = v0 == g.nt[0] - 1 ? 0 : v0 + 1;
+  const int d1 = v1 == 0 ? g.nt[1] - 1 : v1 - 1, u1 = v1 == g.nt[1] - 1 ? 0 : v1 + 1;
+  const int d2 = v2 == 0 ? g.nt[2] - 1 : v2 - 1, u2 = v2 == g.nt[2] - 1 ? 0 : v2 + 1;
+  const int ncell = s0 * s1 * s2;
+  const int n_int = kp.T * ncell;
+  const float icell = 1.0f / (float)ncell, isx = 1.0f / (float)s0, isy = 1.0f / (float)s1;
+  for (int e = threadIdx.x; e < n_int; e += 256) {
+    const int type = small_div(e, icell);
+    const int l = e - type * ncell;
+    const int lyz = small_div(l, isx);
+    const int i0 = l - lyz * s0;
+    const int i2 = small_div(lyz, isy);
+    const int i1 = lyz - i2 * s1;
+    // per axis: option 0 = own window's interior; 1 = high halo of the window
+    // below (cell within h of the low face); 2 = low halo of the window above
+    const bool lo0 = i0 < g.h[0], hi0 = i0 >= s0 - g.h[0];
+    const bool lo1 = i1 < g.h[1], hi1 = i1 >= s1 - g.h[1];
+    const bool lo2 = i2 < g.h[2], hi2 = i2 >= s2 - g.h[2];
+    const uint32_t cell = (uint32_t)type * kp.n_cells +
+                          (uint32_t)((f0 + i0) + kp.nx * ((f1 + i1) + kp.ny * (f2 + i2)));
+    uint32_t sum = BITS == 32 ? counts[cell] : (uint32_t)counts16[cell];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if ((c == 1 && !lo2) || (c == 2 && !hi2)) continue;
+      const int w2 = c == 0 ? v2 : (c == 1 ? d2 : u2);
+      const int r2 = c == 0 ? 1 : (c == 1 ? 2 : 0);
+      const int l2 = c == 2 ? i2 - (s2 - g.h[2]) : i2;
+      const int e2 = c == 0 ? s2 : g.h[2];
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        if ((b == 1 && !lo1) || (b == 2 && !hi1)) continue;
+        const int w1 = b == 0 ? v1 : (b == 1 ? d1 : u1);
+        const int r1 = b == 0 ? 1 : (b == 1 ? 2 : 0);
+        const int l1 = b == 2 ? i1 - (s1 - g.h[1]) : i1;
+        const int e1 = b == 0 ? s1 : g.h[1];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if ((a == 1 && !lo0) || (a == 2 && !hi0)) continue;
+          const int w0 = a == 0 ? v0 : (a == 1 ? d0 : u0);
+          const int r0 = a == 0 ? 1 : (a == 1 ? 2 : 0);
+          const int l0 = a == 2 ? i0 - (s0 - g.h[0]) : i0;
+          const int e0 = a == 0 ? s0 : g.h[0];
+          const int u = w0 + g.nt[0] * (w1 + g.nt[1] * w2);
+          const int64_t idx = (int64_t)u * g.slot_elems + reg_s[r0 + 3 * (r1 + 3 * r2)] +
+                              ((type * e2 + l2) * e1 + l1) * e0 + l0;
+          sum += scratch[idx];
         }
-        out |= sum << (16 * h2);
       }
     }
-    counts[w] = out;
+    if (BITS == 32) {
+      counts[cell] = sum;
+    } else {
+      if (sum > 0xFFFFu) {
+        atomicOr(overflow, 1u);
+        sum = 0xFFFFu;
+      }
+      counts16[cell] = (uint16_t)sum;
+    }
   }
 }
 

@@ -46,6 +46,7 @@ def lib():
     L.orc_init_coords.argtypes = [pD, pO, vp]
     L.orc_sweep.argtypes = [pD, pO, vp, vp, C.c_uint32, C.c_int64, C.c_int64,
                             C.POINTER(C.c_uint64)]
+    L.orc_sweep_injected.argtypes = [pD, pO, vp, vp, vp, vp, vp, C.POINTER(C.c_uint64)]
     L.orc_re2.argtypes = [pD, vp]
     L.orc_re2.restype = C.c_double
     L.orc_philox.argtypes = [C.c_uint32] * 6 + [vp]
@@ -111,6 +112,23 @@ class Oracle:
         acc = C.c_uint64()
         lib().orc_sweep(C.byref(self.desc), C.byref(self.opts), _ptr(pos), _ptr(om),
                         int(sweep), int(chain_begin), int(chain_end), C.byref(acc))
+        return acc.value
+
+    def sweep_injected(self, pos, omega32, pick, gauss, acc_u):
+        """One sweep in place on pos, driven by injected random numbers: move
+        i of the chain whose first bead is b takes pick[b+i] in [0,1] (1 picks the last bead),
+        gauss[b+i] (three normals; three uniforms in [0,1) for trial moves)
+        and acc_u[b+i] in (0,1].  Returns accepted moves."""
+        assert pos.dtype == np.float32 and pos.flags.c_contiguous
+        n = self.n_beads
+        pick = np.ascontiguousarray(pick, np.float64)
+        gauss = np.ascontiguousarray(gauss, np.float32)
+        acc_u = np.ascontiguousarray(acc_u, np.float32)
+        assert pick.size == n and gauss.size == 3 * n and acc_u.size == n
+        om = np.ascontiguousarray(omega32, np.float32)
+        acc = C.c_uint64()
+        lib().orc_sweep_injected(C.byref(self.desc), C.byref(self.opts), _ptr(pos), _ptr(om),
+                                 _ptr(pick), _ptr(gauss), _ptr(acc_u), C.byref(acc))
         return acc.value
 
     def re2(self, pos) -> float:

@@ -194,21 +194,60 @@ inline double bonded_delta(const Sys& s, int p, int mono, int64_t poly_index, co
 // mc_polymer_move_arr, device_function.cu:718-799, for one chain, with
 // trial_move_smc_arr (:637-717), calc_delta_energy_arr (:562-588) and
 // som_accept (:444-454) inlined.  Returns the number of accepted moves.
+// The random numbers of move i of a chain: the Philox streams the engine
+// uses, or arrays injected by the caller (any other generator).
+struct PhiloxDraws {
+  const scmf_opts& o;
+  uint32_t sweep;
+  int64_t chain;
+  u32x4 pick, disp;
+  void at(int i) {
+    pick = stream_block(o.seed, o.system_id, (uint32_t)chain, sweep, (uint32_t)i, SLOT_PICK);
+    disp = stream_block(o.seed, o.system_id, (uint32_t)chain, sweep, (uint32_t)i, SLOT_DISP);
+  }
+  double pick_u() const { return u01_53(pick.v[0], pick.v[1]); }
+  void normals(float* rx, float* ry, float* rz) const { normal3f(disp, rx, ry, rz); }
+  void trial(float* ux, float* uy, float* uz) const {
+    *ux = u01_24f_closed0(disp.v[0]);
+    *uy = u01_24f_closed0(disp.v[1]);
+    *uz = u01_24f_closed0(disp.v[2]);
+  }
+  float accept_u() const { return u01_24f(pick.v[2]); }
+};
+
+// Injected draws, indexed by move: move i of the chain whose first bead is
+// b0 reads pick[b0+i] in [0,1] (1 picks the last bead), gauss[3(b0+i)..] (three normals, or three
+// uniforms in [0,1) for trial moves) and acc[b0+i] in (0,1].
+struct InjectedDraws {
+  const double* pick_;
+  const float* gauss_;
+  const float* acc_;
+  int64_t m = 0;
+  int64_t b0 = 0;
+  void at(int i) { m = b0 + i; }
+  double pick_u() const { return pick_[m]; }
+  void normals(float* rx, float* ry, float* rz) const {
+    *rx = gauss_[3 * m];
+    *ry = gauss_[3 * m + 1];
+    *rz = gauss_[3 * m + 2];
+  }
+  void trial(float* ux, float* uy, float* uz) const { normals(ux, uy, uz); }
+  float accept_u() const { return acc_[m]; }
+};
+
+template <class Draws>
 uint64_t chain_sweep(const Sys& s, const scmf_opts& o, float* pos, const float* omega,
-                     uint32_t sweep, int64_t chain) {
+                     int64_t chain, Draws& rnd) {
   const int p = poly_type_of(s, chain);
   const int64_t poly_index = first_bead(s, p, chain);
   const int length = s.len[p];
   uint64_t acc = 0;
   for (int i = 0; i < length; i++) {
-    const u32x4 pick = stream_block(o.seed, o.system_id, (uint32_t)chain, sweep, (uint32_t)i,
-                                    SLOT_PICK);
-    int ibead = (int)(u01_53(pick.v[0], pick.v[1]) * length);
+    rnd.at(i);
+    int ibead = (int)(rnd.pick_u() * length);
     if (ibead == length) ibead = length - 1;
     const unsigned iwtype = (unsigned)s.monotype[s.mono_off[p] + ibead];
     const int64_t index = poly_index + ibead;
-    const u32x4 disp = stream_block(o.seed, o.system_id, (uint32_t)chain, sweep, (uint32_t)i,
-                                    SLOT_DISP);
     float dx, dy, dz;
     double smc_deltaE = 0.0;
     if (o.move_type == SCMF_MOVE_SMART) {
@@ -218,7 +257,7 @@ uint64_t chain_sweep(const Sys& s, const scmf_opts& o, float* pos, const float* 
       float fx = 0.0f, fy = 0.0f, fz = 0.0f;
       bond_forces(s, p, ibead, poly_index, pos, x, y, z, &fx, &fy, &fz);
       float rx, ry, rz;
-      normal3f(disp, &rx, &ry, &rz);
+      rnd.normals(&rx, &ry, &rz);
       dx = A * fx + rx * R;
       dy = A * fy + ry * R;
       dz = A * fz + rz * R;
@@ -233,9 +272,11 @@ uint64_t chain_sweep(const Sys& s, const scmf_opts& o, float* pos, const float* 
                                 (fz * fz));
     } else {
       const float delta = o.trial_delta;
-      dx = delta * (2.0f * u01_24f_closed0(disp.v[0]) - 1.0f);
-      dy = delta * (2.0f * u01_24f_closed0(disp.v[1]) - 1.0f);
-      dz = delta * (2.0f * u01_24f_closed0(disp.v[2]) - 1.0f);
+      float ux, uy, uz;
+      rnd.trial(&ux, &uy, &uz);
+      dx = delta * (2.0f * ux - 1.0f);
+      dy = delta * (2.0f * uy - 1.0f);
+      dz = delta * (2.0f * uz - 1.0f);
     }
     const float newx = pos[index * 3] + dx;
     const float newy = pos[index * 3 + 1] + dy;
@@ -255,7 +296,7 @@ uint64_t chain_sweep(const Sys& s, const scmf_opts& o, float* pos, const float* 
     delta_energy += smc_deltaE;
     // som_accept: the uniform matters only when p <= 1 (:449).
     const double p_acc = exp_cw(-1.0 * delta_energy);
-    const bool accept = (p_acc > 1) || (p_acc > (double)u01_24f(pick.v[2]));
+    const bool accept = (p_acc > 1) || (p_acc > (double)rnd.accept_u());
     if (accept) {
       pos[index * 3] = newx;
       pos[index * 3 + 1] = newy;
@@ -419,8 +460,29 @@ int orc_sweep(const scmf_desc* d, const scmf_opts* o, float* pos, const float* o
   if (chain_end < 0 || chain_end > s.n_chains) chain_end = s.n_chains;
   uint64_t acc = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : acc)
-  for (int64_t chain = chain_begin; chain < chain_end; ++chain)
-    acc += chain_sweep(s, *o, pos, omega, sweep, chain);
+  for (int64_t chain = chain_begin; chain < chain_end; ++chain) {
+    PhiloxDraws rnd{*o, sweep, chain, {}, {}};
+    acc += chain_sweep(s, *o, pos, omega, chain, rnd);
+  }
+  if (n_accept) *n_accept = acc;
+  return 0;
+}
+
+// One sweep driven by injected random numbers (see InjectedDraws), for
+// statistical comparisons with generators other than the engine's.
+int orc_sweep_injected(const scmf_desc* d, const scmf_opts* o, float* pos, const float* omega,
+                       const double* pick, const float* gauss, const float* acc_u,
+                       uint64_t* n_accept) {
+  Sys s;
+  if (!build(d, s)) return -1;
+  uint64_t acc = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : acc)
+  for (int64_t chain = 0; chain < s.n_chains; ++chain) {
+    const int p = poly_type_of(s, chain);
+    InjectedDraws rnd{pick, gauss, acc_u};
+    rnd.b0 = first_bead(s, p, chain);
+    acc += chain_sweep(s, *o, pos, omega, chain, rnd);
+  }
   if (n_accept) *n_accept = acc;
   return 0;
 }

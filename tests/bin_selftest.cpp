@@ -1,6 +1,7 @@
 // Harness for tests/test_bin_fast.py: compares the engine's fast NGP binning
-// (new_arch_amd/csrc/scmf_bin.h, fold_fast) with the reference computation it
-// must reproduce (fold_exact) on random and adversarial coordinates.
+// (new_arch_amd/csrc/scmf_bin.h, fold and fold_fast) with the reference
+// computation it must reproduce (fold_exact) on random and adversarial
+// coordinates.
 // Usage: bin_selftest L n count seed  -> prints "<checked> <mismatches> <slow>"
 #include <math.h>
 #include <stdio.h>
@@ -21,14 +22,15 @@ int main(int argc, char** argv) {
   auto check = [&](double r) {
     ++checked;
     const int f = scmf::fold_fast(r, a);
+    const int g = scmf::fold(r, a);
     const int e = scmf::fold_exact(r, a.L, a.d, a.n);
-    if (f != e) {
-      if (bad < 10) fprintf(stderr, "mismatch r=%.17g fast=%d exact=%d\n", r, f, e);
+    if (f != e || g != e) {
+      if (bad < 10) fprintf(stderr, "mismatch r=%.17g fast=%d fold=%d exact=%d\n", r, f, g, e);
       ++bad;
     }
   };
   for (long long i = 0; i < count; ++i) {
-    const int kind = (int)(i % 8);
+    const int kind = (int)(i % 11);
     double r;
     if (kind == 0) {
       r = (U(rng) * 2 - 1) * 50 * L;                    // unwrapped coordinates
@@ -56,6 +58,22 @@ int main(int argc, char** argv) {
       r = (double)x + (double)dx;
     } else if (kind == 6) {                             // far away and huge
       r = (U(rng) * 2 - 1) * pow(10.0, U(rng) * 12) * L;
+    } else if (kind == 8) {                             // in-box cell edges +- a few ulp
+      const long long k = (long long)(U(rng) * (n + 1));
+      r = k * a.d;
+      const int s = (int)(U(rng) * 9) - 4;
+      for (int t = 0; t < abs(s); ++t) r = nextafter(r, s > 0 ? INFINITY : -INFINITY);
+    } else if (kind == 9) {                             // around L and L (1 - 2^-50)
+      r = U(rng) < 0.5 ? L : a.lhi;
+      const int s = (int)(U(rng) * 41) - 20;
+      for (int t = 0; t < abs(s); ++t) r = nextafter(r, s > 0 ? INFINITY : -INFINITY);
+      if (U(rng) < 0.5) r = (float)r;
+    } else if (kind == 10) {                            // in-box fp32 cell edges
+      const long long k = (long long)(U(rng) * (n + 1));
+      float f = (float)(k * a.d);
+      const int s = (int)(U(rng) * 7) - 3;
+      for (int t = 0; t < abs(s); ++t) f = nextafterf(f, s > 0 ? INFINITY : -INFINITY);
+      r = f;
     } else {                                            // tiny and signed zeros
       const double v[6] = {0.0, -0.0, 1e-300, -1e-300, 5e-324, -5e-324};
       r = v[i % 6] * (U(rng) < 0.5 ? 1 : 1e10);

@@ -280,34 +280,64 @@ def test_set_rng_replays_a_sweep():
     g.close()
 
 
-def test_large_melt_invariants():
-    """268M beads, 256^3 grid, uint16: size-independent checks of init and gridding."""
-    banner("large diblock melt invariants")
+def full_size_check(spec, bits, type_sums, sweep_steps=1):
+    """Init, gridding, omega and whole sweeps of a full-size system against
+    the oracle on the same positions: counts and positions bit-exact, omega
+    equal to the oracle's fp64 value rounded once; then size-independent
+    invariants (bead sums per type, idempotent gridding, <Re^2> of the walk)."""
+    g = scmf.System(spec, make_opts(seed=0, density_bits=bits))
+    try:
+        o = orc.Oracle(spec, g.opts)
+        g.grid_density()
+        c = g.density()
+        cc = c.reshape(spec.n_types, -1)
+        assert cc.sum(dtype=np.int64) == spec.n_beads
+        for t, n in enumerate(type_sums):
+            assert cc[t].sum(dtype=np.int64) == n
+        assert g.stats().overflow == 0
+        g.grid_density()                          # idempotent
+        assert np.array_equal(g.density(), c)
+        assert abs(g.stats().re2 - 1.0) < 0.01    # Gaussian walks: <Re^2> = 1
+        pos = g.positions()
+        banner("  oracle gridding")
+        assert np.array_equal(o.grid(pos), c)
+        g.update_omega()
+        om = g.omega()
+        banner("  oracle omega")
+        assert same_bits(om, o.omega(c).astype(np.float32))
+        for k in range(sweep_steps):
+            banner(f"  oracle sweep {k}")
+            acc = o.sweep(pos, om, k)
+            g.reset_stats()
+            g.mc_sweep(1)
+            assert g.stats().accepts == acc
+            assert same_bits(g.positions(), pos)
+            g.grid_density()
+            c = g.density()
+            assert np.array_equal(o.grid(pos), c)
+            g.update_omega()
+            om = g.omega()
+    finally:
+        g.close()
+
+
+def test_large_melt_full_size():
+    """268M beads, 256^3 grid, uint16 counts, smart moves: one whole step
+    bit-exact against the oracle."""
+    banner("large diblock melt, full size")
     spec = W.large_diblock_melt()
-    g = scmf.System(spec, make_opts(seed=0, density_bits=16))
-    g.grid_density()
-    c = g.density().reshape(2, -1)
-    assert c.sum() == spec.n_beads
-    assert c[0].sum() == 19 * 4_194_304 and c[1].sum() == 45 * 4_194_304
-    assert g.stats().overflow == 0
-    g.grid_density()                              # idempotent
-    np.testing.assert_array_equal(g.density().reshape(2, -1), c)
-    g.update_omega()
-    om = g.omega()
-    assert np.isfinite(om).all()
-    st = g.stats()
-    assert abs(st.re2 - 1.0) < 0.01               # Gaussian walks: <Re^2> = 1
-    # a spot check of a sweep on a slice: the first 2048 chains against the oracle
-    pos = g.positions()
-    sl = 2048 * 64
-    sub = SystemSpec(n_types=2, xn=spec.xn, L=spec.L, n=spec.n, dtNxi=spec.dtNxi, nref=64,
-                     polymers=[W.diblock(64, 0.3, 2048)])
-    o = orc.Oracle(sub, g.opts)
-    ref = np.ascontiguousarray(pos[:sl])
-    o.sweep(ref, om, 0)
-    g.mc_sweep(1)
-    assert same_bits(g.positions()[:sl], ref)
-    g.close()
+    full_size_check(spec, 16, [19 * 4_194_304, 45 * 4_194_304])
+
+
+def test_largest_blend_full_size():
+    """1.0e9 beads (A/B homopolymers + AB diblocks), 512x512x256 grid, uint32
+    counts, smart moves: one whole step bit-exact against the oracle."""
+    banner("largest blend, full size")
+    spec = W.largest_blend()
+    na = 10_937_500 * 64 + 781_250 * 64
+    nb = 3_125_000 * 64 + 781_250 * 64
+    assert na + nb == spec.n_beads == 1_000_000_000
+    full_size_check(spec, 32, [na, nb])
 
 
 def test_copy_errors():
