@@ -27,6 +27,11 @@
 // farther than 2^-50 (n + 2) from an integer both truncate to the same
 // cell (and q < n, so the mod is the identity).  Everything else takes
 // fold_fast.
+//
+// Power-of-two cells (fold_pow2).  When d = 2^-k exactly, r/d = r 2^k has no
+// rounding in fp64 nor, for fp32 r, in fp32; so for 0 <= r < L (1 - 2^-50)
+// the reference's cell is trunc(r 2^k) whenever that is below n.  The test
+// returns false otherwise and the caller takes fold.
 #pragma once
 #include <stdint.h>
 
@@ -41,7 +46,9 @@ namespace scmf {
 struct Axis {
   double L, d, invL, invd, guard2;  // guard2 = 2^-48 * n
   double lhi, gin, gin1;            // L (1 - 2^-50); 2^-50 (n + 2); 1 - gin
+  float invd_f, lhi_f;              // 1/d and lhi rounded to fp32 (used when pow2)
   int n;
+  int pow2;                         // d is an exact power of two
 };
 
 // The reference's computation, verbatim.  Coordinates it cannot bin
@@ -101,6 +108,23 @@ SCMF_HD int fold(double r, const Axis a) {
   return fold_fast(r, a);
 }
 
+// Power-of-two cells, fp32 coordinate: *i = the reference cell when true.
+// (Every fp32 r < (float)lhi is below lhi itself.)
+SCMF_HD bool fold_pow2(float r, const Axis a, int* i) {
+  const bool in = r >= 0.0f && r < a.lhi_f;
+  const int c = (int)((in ? r : 0.0f) * a.invd_f);
+  *i = c;
+  return in && c < a.n;
+}
+
+// Power-of-two cells, fp64 coordinate.
+SCMF_HD bool fold_pow2(double r, const Axis a, int* i) {
+  const bool in = r >= 0.0 && r < a.lhi;
+  const int c = (int)((in ? r : 0.0) * a.invd);
+  *i = c;
+  return in && c < a.n;
+}
+
 SCMF_HD Axis make_axis(double L, int n) {
   Axis a;
   a.L = L;
@@ -112,6 +136,11 @@ SCMF_HD Axis make_axis(double L, int n) {
   a.lhi = L * (1.0 - 0x1p-50);
   a.gin = 0x1p-50 * ((double)n + 2.0);
   a.gin1 = 1.0 - a.gin;
+  int e = 0;
+  const double m = __builtin_frexp(a.d, &e);
+  a.pow2 = (m == 0.5 && e > -120 && e < 120) ? 1 : 0;
+  a.invd_f = (float)a.invd;
+  a.lhi_f = (float)a.lhi;
   return a;
 }
 

@@ -39,6 +39,7 @@ struct KParams {
   uint32_t n_cells;
   double Lx, Ly, Lz, dx, dy, dz;
   Axis ax[3];                    // binning constants per axis
+  int pow2;                      // every axis has power-of-two cells
   double harm;                   // harmonic_normb
   double inv_nref;               // inverse_refbeads
   double fst[MAX_T];             // field_scaling_type
@@ -118,16 +119,25 @@ __device__ __forceinline__ Axis axis(const KP& kp, int a) {
   r.lhi = kp.ax[a].lhi;
   r.gin = kp.ax[a].gin;
   r.gin1 = kp.ax[a].gin1;
+  r.invd_f = kp.ax[a].invd_f;
+  r.lhi_f = kp.ax[a].lhi_f;
   r.n = kp.ax[a].n;
+  r.pow2 = kp.ax[a].pow2;
   return r;
 }
 
 template <class KP>
 __device__ __forceinline__ uint32_t bin_cell(const KP& kp, double rx, double ry, double rz,
                                              uint32_t type) {
-  const int xi = fold(rx, axis(kp, 0));
-  const int yi = fold(ry, axis(kp, 1));
-  const int zi = fold(rz, axis(kp, 2));
+  const Axis ax = axis(kp, 0), ay = axis(kp, 1), az = axis(kp, 2);
+  int xi, yi, zi;
+  const bool ok = kp.pow2 && (fold_pow2(rx, ax, &xi) & fold_pow2(ry, ay, &yi) &
+                              fold_pow2(rz, az, &zi));
+  if (!ok) {
+    xi = fold(rx, ax);
+    yi = fold(ry, ay);
+    zi = fold(rz, az);
+  }
   return type * kp.n_cells + (uint32_t)(xi + yi * kp.nx + zi * kp.nx * kp.ny);
 }
 

@@ -767,6 +767,7 @@ int derive(scmf_sys* s) {
   kp.ax[0] = make_axis(kp.Lx, kp.nx);
   kp.ax[1] = make_axis(kp.Ly, kp.ny);
   kp.ax[2] = make_axis(kp.Lz, kp.nz);
+  kp.pow2 = kp.ax[0].pow2 && kp.ax[1].pow2 && kp.ax[2].pow2;
   s->n_cells = (int64_t)kp.nx * kp.ny * kp.nz;
   s->T = d.T;
   if ((int64_t)d.T * s->n_cells >= (int64_t)0xFFFFFFFF)
@@ -973,6 +974,8 @@ int launch_sweep(scmf_sys* s) {
 // least window area per interior cell wins (that ratio sets the scratch
 // traffic), with a penalty for one workgroup per CU and for too few tiles to
 // fill the GPU.
+size_t tile_lds_cap() { return (size_t)160 << 10; }
+
 void choose_tiles(scmf_sys* s) {
   const KParams& kp = s->kp;
   s->tile_lds = 0;
@@ -998,14 +1001,17 @@ void choose_tiles(scmf_sys* s) {
   const double d[3] = {kp.dx, kp.dy, kp.dz};
   int hh[3];
   for (int a = 0; a < 3; ++a) hh[a] = std::max(1, std::min(16, (int)ceil(reach / d[a])));
-  const size_t lds_cap = ((size_t)160 << 10) - MONO_LDS - 512;
-  // candidate tile counts per axis: one tile, or pieces no shorter than the
-  // halo and no longer than 64 cells
+  // LDS: the window, the per-axis offset tables, the type table, slack
+  const size_t tables = (size_t)4 * (n[0] + n[1] + n[2]);
+  if (tables > ((size_t)16 << 10)) return;
+  const size_t lds_cap = std::min(((size_t)160 << 10) - MONO_LDS - 512 - tables, tile_lds_cap());
+  // candidate tile counts per axis: one tile, or pieces no shorter than
+  // twice the halo and no longer than 64 cells
   std::vector<int> cand[3];
   for (int a = 0; a < 3; ++a) {
     cand[a].push_back(1);
     for (int nt = 2; nt <= n[a]; ++nt) {
-      if (n[a] / nt < hh[a]) break;
+      if (n[a] / nt < 2 * hh[a]) break;
       if ((n[a] + nt - 1) / nt <= 64) cand[a].push_back(nt);
     }
   }
@@ -1027,10 +1033,11 @@ void choose_tiles(scmf_sys* s) {
           wc *= g.W[a];
           ic *= g.E[a];
         }
-        const size_t bytes = (size_t)((wc * s->T + 1) / 2) * 4;
+        const size_t bytes = (size_t)((((wc * s->T + 1) / 2) + 3) & ~(int64_t)3) * 4;
         if (bytes > lds_cap) continue;
         const int64_t ntiles = (int64_t)n0 * n1 * n2;
-        const int per_cu = (int)std::min<size_t>(2, ((size_t)160 << 10) / (bytes + MONO_LDS + 512));
+        const int per_cu =
+            (int)std::min<size_t>(2, ((size_t)160 << 10) / (bytes + tables + MONO_LDS + 512));
         double cost = (double)wc / (double)ic;
         if (per_cu < 2) cost *= 1.15;
         if (ntiles < 1024) cost *= 1024.0 / (double)ntiles;
@@ -1041,6 +1048,7 @@ void choose_tiles(scmf_sys* s) {
           bg.ntiles = (int)ntiles;
           bg.win_cells = (int)wc;
           bg.win_words = (int)((wc * s->T + 1) / 2);
+          bg.win_words_padded = (bg.win_words + 3) & ~3;
         }
       }
   if (best <= 0) return;
@@ -1055,7 +1063,7 @@ void choose_tiles(scmf_sys* s) {
   }
   bg.slot_elems = (off + 63) & ~63;
   s->geo = bg;
-  s->tile_lds = (size_t)bg.win_words * 4;
+  s->tile_lds = (size_t)bg.win_words_padded * 4 + tables;
 }
 
 int alloc_sort_buffers(scmf_sys* s) {
@@ -1080,13 +1088,22 @@ int alloc_sort_buffers(scmf_sys* s) {
       (rc = A((void**)&s->x2, sizeof(float) * padded(B))) ||
       (rc = A((void**)&s->y2, sizeof(float) * padded(B))) ||
       (rc = A((void**)&s->z2, sizeof(float) * padded(B))) ||
-      (rc = A((void**)&s->scratch, sizeof(uint16_t) * (size_t)s->geo.ntiles * s->geo.slot_elems)))
+      (rc = A((void**)&s->scratch,
+              sizeof(uint16_t) * ((size_t)s->geo.ntiles * s->geo.slot_elems + 64))))
     return rc;
+  // the window sums read absent contributions from the element after the
+  // last image, which nothing writes
+  HIP_TRY(hipMemsetAsync(s->scratch + (size_t)s->geo.ntiles * s->geo.slot_elems, 0,
+                         sizeof(uint16_t) * 64, s->stream));
   if (s->tile_lds + MONO_LDS + 128 > 65536 &&
-      (hipFuncSetAttribute((const void*)k_grid_tiles<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                           (int)s->tile_lds) != hipSuccess ||
-       hipFuncSetAttribute((const void*)k_grid_tiles<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                           (int)s->tile_lds) != hipSuccess)) {
+      (hipFuncSetAttribute((const void*)k_grid_tiles<16, false>,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) != hipSuccess ||
+       hipFuncSetAttribute((const void*)k_grid_tiles<32, false>,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) != hipSuccess ||
+       hipFuncSetAttribute((const void*)k_grid_tiles<16, true>,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) != hipSuccess ||
+       hipFuncSetAttribute((const void*)k_grid_tiles<32, true>,
+                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) != hipSuccess)) {
     (void)hipGetLastError();
     return fail(SCMF_E_HIP, "cannot raise the LDS limit of the tiled gridding");
   }
@@ -1130,17 +1147,14 @@ int sort_chains(scmf_sys* s) {
 // counts must already be cleared).
 int launch_tiled_grid(scmf_sys* s, hipStream_t st, const KParams* Ps, int sys) {
   const dim3 tiles((unsigned)s->geo.ntiles);
-  if (s->opts.density_bits == 32) {
-    hipLaunchKernelGGL(k_grid_tiles<32>, tiles, dim3(GRID_THREADS), s->tile_lds, st, Ps, sys,
-                       s->geo, (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
-    hipLaunchKernelGGL(k_tiles_sum<32>, tiles, dim3(256), 0, st, Ps, sys, s->geo,
-                       (const uint16_t*)s->scratch);
-  } else {
-    hipLaunchKernelGGL(k_grid_tiles<16>, tiles, dim3(GRID_THREADS), s->tile_lds, st, Ps, sys,
-                       s->geo, (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
-    hipLaunchKernelGGL(k_tiles_sum<16>, tiles, dim3(256), 0, st, Ps, sys, s->geo,
-                       (const uint16_t*)s->scratch);
-  }
+  const bool p2 = s->kp.pow2 != 0;
+  auto bin = s->opts.density_bits == 32 ? (p2 ? k_grid_tiles<32, true> : k_grid_tiles<32, false>)
+                                         : (p2 ? k_grid_tiles<16, true> : k_grid_tiles<16, false>);
+  auto sum = s->opts.density_bits == 32 ? k_tiles_sum<32> : k_tiles_sum<16>;
+  hipLaunchKernelGGL(bin, tiles, dim3(GRID_THREADS), s->tile_lds, st, Ps, sys, s->geo,
+                     (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
+  hipLaunchKernelGGL(sum, tiles, dim3(SUM_THREADS), 0, st, Ps, sys, s->geo,
+                     (const uint16_t*)s->scratch);
   HIP_TRY(hipGetLastError());
   return SCMF_OK;
 }

@@ -18,8 +18,9 @@
 // on how well ordered the chains are; only speed does.
 //
 // Tiles split each axis into nt pieces of q or q+1 cells (the first `rem`
-// pieces get q+1); a piece is never shorter than the halo, so every halo
-// region of a window lies inside one neighbouring tile.
+// pieces get q+1); a piece is never shorter than twice the halo, so every
+// halo region of a window lies inside one neighbouring tile and every cell
+// lies in the halo of at most one neighbour per axis.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,6 +40,7 @@ struct TileGeo {
   int ntiles;         // nt0 nt1 nt2
   int win_cells;      // W0 W1 W2
   int win_words;      // T win_cells / 2, rounded up (uint16 pairs)
+  int win_words_padded;  // win_words rounded up to a multiple of 4
   int slot_elems;     // uint16 elements of one window image in scratch
   int reg_off[27];    // element offset of region rx + 3 ry + 9 rz in an image
 };
@@ -175,10 +177,25 @@ __device__ __forceinline__ int wrap_rel(int c, int origin, int n) {
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int MONO_LDS = 4096;     // architectures up to this length read types from LDS
+constexpr uint32_t OUT_OF_WINDOW = 1u << 30;  // the sum of three table entries stays below 2^32
 constexpr int GRID_THREADS = 1024;
-// Beads binned between two normalisations of the window: each uint16 half
-// then grows by at most this much from below 2^15, so it cannot wrap.
-constexpr int NORM_BEADS = 7 * 4 * GRID_THREADS;
+// A pass of the binning loop: every lane bins four beads.
+constexpr int PASS_BEADS = 4 * GRID_THREADS;
+// Beads binned between two normalisations of the window (whole passes, at
+// most 32767): each uint16 half then grows by at most this much from below
+// 2^15, so it cannot wrap.
+constexpr int NORM_BEADS = (32767 / PASS_BEADS) * PASS_BEADS;
+static_assert(NORM_BEADS > 0 && NORM_BEADS <= 32767, "normalisation interval");
+
+// The general binning of three coordinates (the tiled gridding needs it only
+// for beads outside the box or off power-of-two cells).
+__device__ __forceinline__ void fold3_general(float x, float y, float z, const Axis ax,
+                                           const Axis ay, const Axis az, int* xi, int* yi,
+                                           int* zi) {
+  *xi = fold((double)x, ax);
+  *yi = fold((double)y, ay);
+  *zi = fold((double)z, az);
+}
 
 // Counts `v` more beads of global cell `cell` with a global atomic.
 template <int BITS>
@@ -198,7 +215,12 @@ __device__ __forceinline__ void add_global(SCMF_GLOBAL uint32_t* counts, uint32_
 template <int BITS>
 __device__ void normalise_window(uint32_t* win, const TileGeo& g, KPC& kp, int ox, int oy, int oz,
                                  SCMF_GLOBAL uint32_t* counts, uint32_t* overflow) {
-  for (int w = threadIdx.x; w < g.win_words; w += GRID_THREADS) {
+  // four words per lane and test (the window is padded to a multiple of four)
+  for (int w4 = threadIdx.x; 4 * w4 < g.win_words; w4 += GRID_THREADS) {
+    const uint4 q = ((const uint4*)win)[w4];
+    if (!((q.x | q.y | q.z | q.w) & 0x80008000u)) continue;
+    for (int j = 0; j < 4; ++j) {
+    const int w = 4 * w4 + j;
     const uint32_t v = win[w];
     if (!(v & 0x80008000u)) continue;
     uint32_t keep = v;
@@ -216,6 +238,7 @@ __device__ void normalise_window(uint32_t* win, const TileGeo& g, KPC& kp, int o
                        (uint32_t)type * kp.n_cells + (uint32_t)(cx + kp.nx * (cy + kp.ny * cz)), val);
     }
     win[w] = keep;
+    }
   }
 }
 
@@ -224,7 +247,7 @@ __device__ void normalise_window(uint32_t* win, const TileGeo& g, KPC& kp, int o
 // straight to the final counts with a global atomic.  Positions are read
 // four beads per lane (16-byte loads; the arrays are padded to a multiple of
 // four beads), the next four in flight while the current ones are binned.
-template <int BITS>
+template <int BITS, bool POW2>
 __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __restrict__ Ps, int sys,
                                                              TileGeo g,
                                                              const uint32_t* __restrict__ tile_start,
@@ -235,6 +258,11 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   __shared__ int reg_s[27];
   KPC& kp = param(Ps, (uint32_t)sys);
   if (threadIdx.x < 27) reg_s[threadIdx.x] = g.reg_off[threadIdx.x];
+  // per-axis tables after the window: cell -> offset in the window (scaled
+  // by the strides of the window box), or OUT_OF_WINDOW
+  uint32_t* const tab_x = win + g.win_words_padded;
+  uint32_t* const tab_y = tab_x + kp.nx;
+  uint32_t* const tab_z = tab_y + kp.ny;
   const SCMF_GLOBAL float* __restrict__ x = gptr((const float*)kp.x);
   const SCMF_GLOBAL float* __restrict__ y = gptr((const float*)kp.y);
   const SCMF_GLOBAL float* __restrict__ z = gptr((const float*)kp.z);
@@ -249,7 +277,19 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
             oy = tile_first(g.q[1], g.rem[1], ty) - g.h[1],
             oz = tile_first(g.q[2], g.rem[2], tz) - g.h[2];
   const int wxm = sx + 2 * g.h[0], wym = sy + 2 * g.h[1], wzm = sz + 2 * g.h[2];
-  for (int w = threadIdx.x; w < g.win_words; w += GRID_THREADS) win[w] = 0u;
+  for (int w = threadIdx.x; w < g.win_words_padded; w += GRID_THREADS) win[w] = 0u;
+  for (int c = threadIdx.x; c < kp.nx; c += GRID_THREADS) {
+    const int d = wrap_rel(c, ox, kp.nx);
+    tab_x[c] = d < wxm ? (uint32_t)d : OUT_OF_WINDOW;
+  }
+  for (int c = threadIdx.x; c < kp.ny; c += GRID_THREADS) {
+    const int d = wrap_rel(c, oy, kp.ny);
+    tab_y[c] = d < wym ? (uint32_t)(d * g.W[0]) : OUT_OF_WINDOW;
+  }
+  for (int c = threadIdx.x; c < kp.nz; c += GRID_THREADS) {
+    const int d = wrap_rel(c, oz, kp.nz);
+    tab_z[c] = d < wzm ? (uint32_t)(d * g.W[0] * g.W[1]) : OUT_OF_WINDOW;
+  }
   int64_t pop = 0;
   for (int p = 0; p < kp.P; ++p)
     pop += (int64_t)(tile_start[p * g.ntiles + t + 1] - tile_start[p * g.ntiles + t]) * kp.len[p];
@@ -274,46 +314,56 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
     // The type source is fixed per architecture; two copies of the loop keep
     // a global type load from sharing registers with the LDS one (which
     // would make every LDS read wait for the prefetched positions).
+    // The type source is fixed per architecture; two copies of the loop keep
+    // a global type load from sharing registers with the LDS one (which
+    // would make every LDS read wait for the prefetched positions).
     auto bin_range = [&](auto tab_c) {
       constexpr bool TAB = decltype(tab_c)::value;
-      // Loads are issued unconditionally (lanes past the range re-read the
-      // last group of the range), so the wait before binning counts only
-      // the current group and the next group stays in flight.
+      // Each lane takes four beads per pass and loads the next pass's four
+      // before binning these.  Loads are issued unconditionally (lanes past
+      // the range re-read the last group of the range), so the wait before
+      // binning counts only the current group and the next stays in flight.
       const int64_t last = (b1 - 1) & ~(int64_t)3;
       int64_t gq = (b0 & ~(int64_t)3) + 4 * (int64_t)threadIdx.x;
+      // monomer index of the lane's first bead (floor mod len), then advanced
+      const int step_m = PASS_BEADS % len;
+      int m0;
+      {
+        const int64_t rel = gq - bb;
+        const int64_t qd = (int64_t)__builtin_floor((double)rel * inv_len);
+        m0 = (int)(rel - qd * len);
+        if (m0 < 0) m0 += len;
+        if (m0 >= len) m0 -= len;
+      }
       const int64_t a0 = gq < b1 ? gq : last;
       f32x4 X = *(const SCMF_GLOBAL f32x4*)(x + a0);
       f32x4 Y = *(const SCMF_GLOBAL f32x4*)(y + a0);
       f32x4 Z = *(const SCMF_GLOBAL f32x4*)(z + a0);
-      for (int64_t base = b0 & ~(int64_t)3; base < b1; base += 4 * GRID_THREADS) {
-        const int64_t gn = gq + 4 * GRID_THREADS;
+      for (int64_t base = b0 & ~(int64_t)3; base < b1; base += PASS_BEADS) {
+        const int64_t gn = gq + PASS_BEADS;
         const int64_t an = gn < b1 ? gn : last;
         const f32x4 Xn = *(const SCMF_GLOBAL f32x4*)(x + an);
         const f32x4 Yn = *(const SCMF_GLOBAL f32x4*)(y + an);
         const f32x4 Zn = *(const SCMF_GLOBAL f32x4*)(z + an);
         if (gq < b1) {
-          // monomer index of the first bead of the four (floor mod len)
-          const int64_t rel = gq - bb;
-          const int64_t qd = (int64_t)__builtin_floor((double)rel * inv_len);
-          int m = (int)(rel - qd * len);
-          if (m < 0) m += len;
-          if (m >= len) m -= len;
+          int m = m0;
           const float px[4] = {X.x, X.y, X.z, X.w}, py[4] = {Y.x, Y.y, Y.z, Y.w},
                       pz[4] = {Z.x, Z.y, Z.z, Z.w};
+          // beads e in [elo, ehi) of the four belong to the range
+          const int elo = gq < b0 ? (int)(b0 - gq) : 0;
+          const int ehi = b1 - gq < 4 ? (int)(b1 - gq) : 4;
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            const int64_t gb = gq + e;
-            if (gb >= b0 && gb < b1) {
-              const uint32_t ty_ = TAB ? (uint32_t)mono_s[m] : (uint32_t)btype[gb];
-              const int xi = fold((double)px[e], ax0);
-              const int yi = fold((double)py[e], ax1);
-              const int zi = fold((double)pz[e], ax2);
-              const int wx = wrap_rel(xi, ox, kp.nx), wy = wrap_rel(yi, oy, kp.ny),
-                        wz = wrap_rel(zi, oz, kp.nz);
-              if (wx < wxm && wy < wym && wz < wzm) {
-                const uint32_t wc =
-                    ty_ * (uint32_t)g.win_cells + (uint32_t)(wx + g.W[0] * (wy + g.W[1] * wz));
-                atomicAdd(&win[wc >> 1], 1u << ((wc & 1u) * 16u));
+            if (e >= elo && e < ehi) {
+              const uint32_t ty_ = TAB ? (uint32_t)mono_s[m] : (uint32_t)btype[gq + e];
+              int xi, yi, zi;
+              const bool quick = POW2 && (fold_pow2(px[e], ax0, &xi) & fold_pow2(py[e], ax1, &yi) &
+                                          fold_pow2(pz[e], ax2, &zi));
+              if (!quick) fold3_general(px[e], py[e], pz[e], ax0, ax1, ax2, &xi, &yi, &zi);
+              const uint32_t sp = tab_x[xi] + tab_y[yi] + tab_z[zi];
+              if (sp < OUT_OF_WINDOW) {
+                const uint32_t wc = __umul24(ty_, (uint32_t)g.win_cells) + sp;
+                atomicAdd(&win[wc >> 1], 1u << ((wc & 1u) << 4));
               } else {
                 ++missed;
                 add_global<BITS>(counts, overflow,
@@ -327,7 +377,9 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
         Y = Yn;
         Z = Zn;
         gq = gn;
-        if (checked && (since += 4 * GRID_THREADS) >= NORM_BEADS) {
+        m0 += step_m;
+        if (m0 >= len) m0 -= len;
+        if (checked && (since += PASS_BEADS) >= NORM_BEADS) {
           since = 0;
           __syncthreads();
           normalise_window<BITS>(win, g, kp, ox, oy, oz, counts, overflow);
@@ -359,16 +411,19 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
     if (n_r == 0) continue;
     const float iex = 1.0f / (float)ex, iey = 1.0f / (float)ey, irow = 1.0f / (float)nrow;
     uint16_t* __restrict__ dst = img + reg_s[r];
+    // (every product below is of values under 2^24)
+    const uint32_t W0 = (uint32_t)g.W[0], W01 = (uint32_t)(g.W[0] * g.W[1]), wcl = (uint32_t)g.win_cells;
+    const uint32_t base_off = (uint32_t)fx + W0 * (uint32_t)fy + W01 * (uint32_t)fz;
     for (int e = threadIdx.x; e < n_r; e += GRID_THREADS) {
       const int type = small_div(e, irow);
-      const int l = e - type * nrow;
+      const int l = e - (int)__umul24((uint32_t)type, (uint32_t)nrow);
       const int lyz = small_div(l, iex);
-      const int lx = l - lyz * ex;
+      const int lx = l - (int)__umul24((uint32_t)lyz, (uint32_t)ex);
       const int lz = small_div(lyz, iey);
-      const int ly = lyz - lz * ey;
-      const uint32_t wc = (uint32_t)type * (uint32_t)g.win_cells +
-                          (uint32_t)((fx + lx) + g.W[0] * ((fy + ly) + g.W[1] * (fz + lz)));
-      dst[e] = (uint16_t)(win[wc >> 1] >> ((wc & 1u) * 16u));
+      const int ly = lyz - (int)__umul24((uint32_t)lz, (uint32_t)ey);
+      const uint32_t wc = __umul24((uint32_t)type, wcl) + base_off + (uint32_t)lx +
+                          __umul24(W0, (uint32_t)ly) + __umul24(W01, (uint32_t)lz);
+      dst[e] = (uint16_t)(win[wc >> 1] >> ((wc & 1u) << 4));
     }
   }
 }
@@ -376,8 +431,18 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
 // One workgroup per tile: every cell of the tile gets its window's interior
 // value, the halo values of the neighbouring windows that reach it, and the
 // globally counted beads already in the counts.
+constexpr int SUM_THREADS = 1024;
+
+// Workgroups are dealt to the 8 XCDs in turn; give each XCD a contiguous run
+// of tiles so that neighbouring tiles (which share cache lines of the counts)
+// meet in the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int x = b & 7, i = b >> 3, per = n >> 3, extra = n & 7;
+  return x * per + (x < extra ? x : extra) + i;
+}
+
 template <int BITS>
-__global__ void __launch_bounds__(256) k_tiles_sum(const KParams* __restrict__ Ps, int sys,
+__global__ void __launch_bounds__(SUM_THREADS) k_tiles_sum(const KParams* __restrict__ Ps, int sys,
                                                    TileGeo g,
                                                    const uint16_t* __restrict__ scratch) {
   KPC& kp = param(Ps, (uint32_t)sys);
@@ -387,7 +452,7 @@ __global__ void __launch_bounds__(256) k_tiles_sum(const KParams* __restrict__ P
   __shared__ int reg_s[27];
   if (threadIdx.x < 27) reg_s[threadIdx.x] = g.reg_off[threadIdx.x];
   __syncthreads();
-  const int t = blockIdx.x;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
   const int v0 = t % g.nt[0], v1 = (t / g.nt[0]) % g.nt[1], v2 = t / (g.nt[0] * g.nt[1]);
   const int s0 = tile_len(g.q[0], g.rem[0], v0), s1 = tile_len(g.q[1], g.rem[1], v1),
             s2 = tile_len(g.q[2], g.rem[2], v2);
@@ -400,49 +465,42 @@ __global__ void __launch_bounds__(256) k_tiles_sum(const KParams* __restrict__ P
   const int ncell = s0 * s1 * s2;
   const int n_int = kp.T * ncell;
   const float icell = 1.0f / (float)ncell, isx = 1.0f / (float)s0, isy = 1.0f / (float)s1;
-  for (int e = threadIdx.x; e < n_int; e += 256) {
+  const int64_t zero_idx = (int64_t)g.ntiles * g.slot_elems;  // an element that stays 0
+  for (int e = threadIdx.x; e < n_int; e += SUM_THREADS) {
     const int type = small_div(e, icell);
     const int l = e - type * ncell;
     const int lyz = small_div(l, isx);
     const int i0 = l - lyz * s0;
     const int i2 = small_div(lyz, isy);
     const int i1 = lyz - i2 * s1;
-    // per axis: option 0 = own window's interior; 1 = high halo of the window
-    // below (cell within h of the low face); 2 = low halo of the window above
+    // per axis: the own window's interior, and at most one neighbouring
+    // window whose halo covers the cell (the window below if the cell is in
+    // the low band, above if in the high band)
     const bool lo0 = i0 < g.h[0], hi0 = i0 >= s0 - g.h[0];
     const bool lo1 = i1 < g.h[1], hi1 = i1 >= s1 - g.h[1];
     const bool lo2 = i2 < g.h[2], hi2 = i2 >= s2 - g.h[2];
+    const int wn0 = lo0 ? d0 : u0, rn0 = lo0 ? 2 : 0, ln0 = lo0 ? i0 : i0 - (s0 - g.h[0]);
+    const int wn1 = lo1 ? d1 : u1, rn1 = lo1 ? 2 : 0, ln1 = lo1 ? i1 : i1 - (s1 - g.h[1]);
+    const int wn2 = lo2 ? d2 : u2, rn2 = lo2 ? 2 : 0, ln2 = lo2 ? i2 : i2 - (s2 - g.h[2]);
+    const bool has0 = lo0 || hi0, has1 = lo1 || hi1, has2 = lo2 || hi2;
     const uint32_t cell = (uint32_t)type * kp.n_cells +
                           (uint32_t)((f0 + i0) + kp.nx * ((f1 + i1) + kp.ny * (f2 + i2)));
+    int64_t idx[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const bool a = c & 1, b = c & 2, z = c & 4;
+      const int w0 = a ? wn0 : v0, r0 = a ? rn0 : 1, l0 = a ? ln0 : i0, e0 = a ? g.h[0] : s0;
+      const int w1 = b ? wn1 : v1, r1 = b ? rn1 : 1, l1 = b ? ln1 : i1, e1 = b ? g.h[1] : s1;
+      const int w2 = z ? wn2 : v2, r2 = z ? rn2 : 1, l2 = z ? ln2 : i2, e2 = z ? g.h[2] : s2;
+      const bool live = (!a || has0) && (!b || has1) && (!z || has2);
+      const int u = w0 + g.nt[0] * (w1 + g.nt[1] * w2);
+      idx[c] = live ? (int64_t)u * g.slot_elems + reg_s[r0 + 3 * (r1 + 3 * r2)] +
+                          ((type * e2 + l2) * e1 + l1) * e0 + l0
+                    : zero_idx;
+    }
     uint32_t sum = BITS == 32 ? counts[cell] : (uint32_t)counts16[cell];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      if ((c == 1 && !lo2) || (c == 2 && !hi2)) continue;
-      const int w2 = c == 0 ? v2 : (c == 1 ? d2 : u2);
-      const int r2 = c == 0 ? 1 : (c == 1 ? 2 : 0);
-      const int l2 = c == 2 ? i2 - (s2 - g.h[2]) : i2;
-      const int e2 = c == 0 ? s2 : g.h[2];
-#pragma unroll
-      for (int b = 0; b < 3; ++b) {
-        if ((b == 1 && !lo1) || (b == 2 && !hi1)) continue;
-        const int w1 = b == 0 ? v1 : (b == 1 ? d1 : u1);
-        const int r1 = b == 0 ? 1 : (b == 1 ? 2 : 0);
-        const int l1 = b == 2 ? i1 - (s1 - g.h[1]) : i1;
-        const int e1 = b == 0 ? s1 : g.h[1];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          if ((a == 1 && !lo0) || (a == 2 && !hi0)) continue;
-          const int w0 = a == 0 ? v0 : (a == 1 ? d0 : u0);
-          const int r0 = a == 0 ? 1 : (a == 1 ? 2 : 0);
-          const int l0 = a == 2 ? i0 - (s0 - g.h[0]) : i0;
-          const int e0 = a == 0 ? s0 : g.h[0];
-          const int u = w0 + g.nt[0] * (w1 + g.nt[1] * w2);
-          const int64_t idx = (int64_t)u * g.slot_elems + reg_s[r0 + 3 * (r1 + 3 * r2)] +
-                              ((type * e2 + l2) * e1 + l1) * e0 + l0;
-          sum += scratch[idx];
-        }
-      }
-    }
+    for (int c = 0; c < 8; ++c) sum += scratch[idx[c]];
     if (BITS == 32) {
       counts[cell] = sum;
     } else {

@@ -24,8 +24,17 @@ int main(int argc, char** argv) {
     const int f = scmf::fold_fast(r, a);
     const int g = scmf::fold(r, a);
     const int e = scmf::fold_exact(r, a.L, a.d, a.n);
-    if (f != e || g != e) {
-      if (bad < 10) fprintf(stderr, "mismatch r=%.17g fast=%d fold=%d exact=%d\n", r, f, g, e);
+    int pd = e, pf = e;
+    if (a.pow2) {  // power-of-two path: when it claims a cell, the cell is right
+      int c;
+      if (scmf::fold_pow2(r, a, &c)) pd = c;
+      const float rf = (float)r;
+      if (scmf::fold_pow2(rf, a, &c) && c != scmf::fold_exact((double)rf, a.L, a.d, a.n)) pf = -1;
+    }
+    if (f != e || g != e || pd != e || pf != e) {
+      if (bad < 10)
+        fprintf(stderr, "mismatch r=%.17g fast=%d fold=%d pow2=%d pow2f=%d exact=%d\n", r, f, g,
+                pd, pf, e);
       ++bad;
     }
   };
@@ -82,6 +91,10 @@ int main(int argc, char** argv) {
   }
   const double special[5] = {INFINITY, -INFINITY, NAN, 1e300, -1e300};
   for (double r : special) check(r);
+  if (argc > 5 && atoi(argv[5]) != a.pow2) {
+    fprintf(stderr, "pow2 flag %d, expected %s\n", a.pow2, argv[5]);
+    ++bad;
+  }
   printf("%lld %lld\n", checked, bad);
   return bad != 0;
 }

@@ -22,10 +22,12 @@ def harness(tmp_path_factory):
     return exe
 
 
-@pytest.mark.parametrize("L,n", [(4.0, 24), (4.0, 32), (32.0, 256), (64.0, 512), (1.0, 6),
-                                 (3.0, 12), (2.5, 10), (0.7, 7), (100.0, 1000)])
-def test_fast_binning_equals_reference(harness, L, n):
-    r = subprocess.run([harness, str(L), str(n), "2000000", str(n * 7 + 1)],
+@pytest.mark.parametrize("L,n,pow2", [(4.0, 24, 0), (4.0, 32, 1), (32.0, 256, 1), (64.0, 512, 1),
+                                      (1.0, 6, 0), (3.0, 12, 1), (2.5, 10, 1), (0.7, 7, 0),
+                                      (100.0, 1000, 0), (2.0, 16, 1), (0.5, 64, 1),
+                                      (1024.0, 64, 1), (3.0, 24, 1), (6.0, 3, 1)])
+def test_fast_binning_equals_reference(harness, L, n, pow2):
+    r = subprocess.run([harness, str(L), str(n), "2000000", str(n * 7 + 1), str(pow2)],
                        capture_output=True, text=True)
     checked, bad = map(int, r.stdout.split())
     assert bad == 0, r.stderr
