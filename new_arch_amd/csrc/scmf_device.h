@@ -126,19 +126,31 @@ __device__ __forceinline__ Axis axis(const KP& kp, int a) {
   return r;
 }
 
+// Linear index x + nx (y + ny z) of a cell (device_function.cuh:19).
 template <class KP>
-__device__ __forceinline__ uint32_t bin_cell(const KP& kp, double rx, double ry, double rz,
-                                             uint32_t type) {
+__device__ __forceinline__ uint32_t cell_index(const KP& kp, int xi, int yi, int zi) {
+  return (uint32_t)(xi + yi * kp.nx + zi * kp.nx * kp.ny);
+}
+
+// NGP cell (within one type's field) of a position, coordinates given in
+// fp32 (a stored position) or fp64 (a sum the reference forms in fp64).
+template <class KP, class R>
+__device__ __forceinline__ uint32_t cell_of(const KP& kp, R rx, R ry, R rz) {
   const Axis ax = axis(kp, 0), ay = axis(kp, 1), az = axis(kp, 2);
   int xi, yi, zi;
   const bool ok = kp.pow2 && (fold_pow2(rx, ax, &xi) & fold_pow2(ry, ay, &yi) &
                               fold_pow2(rz, az, &zi));
   if (!ok) {
-    xi = fold(rx, ax);
-    yi = fold(ry, ay);
-    zi = fold(rz, az);
+    xi = fold((double)rx, ax);
+    yi = fold((double)ry, ay);
+    zi = fold((double)rz, az);
   }
-  return type * kp.n_cells + (uint32_t)(xi + yi * kp.nx + zi * kp.nx * kp.ny);
+  return cell_index(kp, xi, yi, zi);
+}
+
+template <class KP, class R>
+__device__ __forceinline__ uint32_t bin_cell(const KP& kp, R rx, R ry, R rz, uint32_t type) {
+  return type * kp.n_cells + cell_of(kp, rx, ry, rz);
 }
 
 // polymer_number_to_index (device_function.cu:72-93): architecture of a chain.

@@ -184,11 +184,11 @@ __device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, int p, in
   }
   const float newx = ox + dx, newy = oy + dy, newz = oz + dz;
   const double xold = ox, yold = oy, zold = oz;
-  const uint32_t cell_old = bin_cell(kp, xold, yold, zold, iwtype);
-  const uint32_t cell_new =
-      bin_cell(kp, xold + (double)dx, yold + (double)dy, zold + (double)dz, iwtype);
-  const double energy_old = omega[cell_old];
-  const double energy_new = omega[cell_new];
+  // old cell from the stored fp32 position, new cell from the fp64 sum (:573-577)
+  const uint32_t cell_old = cell_of(kp, ox, oy, oz);
+  const uint32_t cell_new = cell_of(kp, xold + (double)dx, yold + (double)dy, zold + (double)dz);
+  const double energy_old = omega[iwtype * kp.n_cells + cell_old];
+  const double energy_new = omega[iwtype * kp.n_cells + cell_new];
   double bonded = 0;
   {
     const double ddx = dx, ddy = dy, ddz = dz;
@@ -311,8 +311,9 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
 //
 // LDS per chain (index type I: uint8 for N <= 255): positions 3N floats
 // (bead b at [b*G + slot]); lev[N] (pick, then level of each move);
-// W[N+1] (level per bead during the scan, then per-level fill counts);
-// start[N+2] (first list entry of each level); list[N] (moves by level).
+// W[N+2] (level per bead during the scan, bead b at W[b+1] with zero
+// sentinels around, then per-level fill counts); start[N+2] (first list
+// entry of each level); list[N] (moves by level).
 template <int L, typename I>
 __global__ void __launch_bounds__(64) k_sweep_sched(const KParams* __restrict__ Ps,
                                                     const BlockRef* __restrict__ bt, int nmax) {
@@ -331,10 +332,10 @@ __global__ void __launch_bounds__(64) k_sweep_sched(const KParams* __restrict__ 
   float* sy = lds + nmax * G;
   float* sz = lds + 2 * nmax * G;
   I* book = (I*)(lds + 3 * nmax * G);
-  const int per = 4 * nmax + 3;
+  const int per = 4 * nmax + 4;
   I* lev = book + slot * per;
   I* W = lev + nmax;
-  I* start = W + nmax + 1;
+  I* start = W + nmax + 2;
   I* list = start + nmax + 2;
   const int64_t chain = (int64_t)br.local * G + slot;
   const bool active = chain < kp.n_chains;
@@ -359,21 +360,34 @@ __global__ void __launch_bounds__(64) k_sweep_sched(const KParams* __restrict__ 
     if (ib == length) ib = length - 1;
     lev[i] = (I)ib;
   }
-  for (int k = r; k <= length; k += L) W[k] = 0;
+  for (int k = r; k <= length + 1; k += L) W[k] = 0;
   for (int k = r; k <= length + 1; k += L) start[k] = 0;
   __syncthreads();
   // dependency levels (one lane per chain), level sizes, moves listed by level
   if (r == 0 && active) {
+    I* const Wb = W + 1;  // level of each bead, with zero sentinels at beads -1 and N
     int maxl = 0;
-    for (int i = 0; i < length; ++i) {
-      const int b = lev[i];
-      int l = W[b];
-      for_neighbours(kp, p, b, length, [&](int j) { l = max(l, (int)W[j]); });
-      l += 1;
-      W[b] = (I)l;
-      lev[i] = (I)l;
-      start[l + 1] = (I)(start[l + 1] + 1);
-      maxl = max(maxl, l);
+    if (kp.linear[p]) {
+      // bonds i <-> i+1: the neighbourhood of b is b-1, b, b+1 (sentinels at the ends)
+      for (int i = 0; i < length; ++i) {
+        const int b = lev[i];
+        const int l = max(max((int)Wb[b - 1], (int)Wb[b]), (int)Wb[b + 1]) + 1;
+        Wb[b] = (I)l;
+        lev[i] = (I)l;
+        start[l + 1] = (I)(start[l + 1] + 1);
+        maxl = max(maxl, l);
+      }
+    } else {
+      for (int i = 0; i < length; ++i) {
+        const int b = lev[i];
+        int l = Wb[b];
+        for_neighbours(kp, p, b, length, [&](int j) { l = max(l, (int)Wb[j]); });
+        l += 1;
+        Wb[b] = (I)l;
+        lev[i] = (I)l;
+        start[l + 1] = (I)(start[l + 1] + 1);
+        maxl = max(maxl, l);
+      }
     }
     for (int l = 1; l <= maxl; ++l) {
       start[l + 1] = (I)(start[l + 1] + start[l]);
@@ -415,7 +429,7 @@ __global__ void __launch_bounds__(64) k_sweep_sched(const KParams* __restrict__ 
 template <int L, typename I>
 size_t sched_lds_bytes(int nmax) {
   constexpr int G = 64 / L;
-  return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (4 * nmax + 3) * sizeof(I);
+  return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (4 * nmax + 4) * sizeof(I);
 }
 
 // Clear each system's counts (a kernel, not a memset, so it can sit in a graph).
@@ -446,7 +460,7 @@ __global__ void __launch_bounds__(BLOCK) k_grid(const KParams* __restrict__ Ps,
   const int64_t stride = (int64_t)br.nblk * BLOCK;
   for (int64_t g = (int64_t)br.local * BLOCK + threadIdx.x; g < kp.n_beads; g += stride) {
     const uint32_t t = gptr((const uint8_t*)kp.btype)[g];
-    const uint32_t cell = bin_cell(kp, (double)x[g], (double)y[g], (double)z[g], t);
+    const uint32_t cell = bin_cell(kp, x[g], y[g], z[g], t);
     if (b32) {
       atomicAdd((uint32_t*)&words[cell], 1u);
     } else {
