@@ -120,40 +120,79 @@ __device__ __forceinline__ void for_neighbours(const KP& kp, int p, int b, int l
   }
 }
 
-// add_bond_forces_arr, device_function.cu:589-634: neighbour difference in
-// fp32, scaled and accumulated in fp64, added to the fp32 force.
+constexpr int MAX_DEG = 8;  // bonds per bead handled by the sweep kernels
+
+// Bonded neighbours of a bead, in ascending bead index.  For linear chains
+// (bonds i <-> i+1) the two possible neighbours are read once per move into
+// registers; otherwise they are read where used.
+template <bool LINEAR, class KP, class V>
+struct Neighbours;
+
 template <class KP, class V>
-__device__ __forceinline__ void bond_force(const KP& kp, const V& c, int p, int b, int len,
-                                           float px, float py, float pz, float* fx, float* fy,
-                                           float* fz) {
-  const double harm = kp.harm;
+struct Neighbours<true, KP, V> {
+  bool hl, hr;  // bead b-1, bead b+1 exist
+  float lx, ly, lz, rx, ry, rz;
+  __device__ Neighbours(const KP&, const V& c, int, int b, int len) : hl(b > 0), hr(b < len - 1) {
+    lx = hl ? c.X(b - 1) : 0.0f;
+    ly = hl ? c.Y(b - 1) : 0.0f;
+    lz = hl ? c.Z(b - 1) : 0.0f;
+    rx = hr ? c.X(b + 1) : 0.0f;
+    ry = hr ? c.Y(b + 1) : 0.0f;
+    rz = hr ? c.Z(b + 1) : 0.0f;
+  }
+  template <class F>
+  __device__ void each(F&& f) const {
+    if (hl) f(lx, ly, lz);
+    if (hr) f(rx, ry, rz);
+  }
+};
+
+template <class KP, class V>
+struct Neighbours<false, KP, V> {
+  const KP& kp;
+  const V& c;
+  int p, b, len;
+  __device__ Neighbours(const KP& kp_, const V& c_, int p_, int b_, int len_)
+      : kp(kp_), c(c_), p(p_), b(b_), len(len_) {}
+  template <class F>
+  __device__ void each(F&& f) const {
+    for_neighbours(kp, p, b, len, [&](int j) { f(c.X(j), c.Y(j), c.Z(j)); });
+  }
+};
+
+// add_bond_forces_arr, device_function.cu:589-634: neighbour difference in
+// fp32, scaled and accumulated in fp64, added to the fp32 force.  (d * 2.0)
+// * harm rounds exactly as d * (2 harm), doubling being exact.
+template <class N>
+__device__ __forceinline__ void bond_force(const N& nb, double harm2, float px, float py, float pz,
+                                           float* fx, float* fy, float* fz) {
   double v1x = 0.0, v1y = 0.0, v1z = 0.0;
-  for_neighbours(kp, p, b, len, [&](int j) {
-    v1x += (double)(c.X(j) - px) * 2.0 * harm;
-    v1y += (double)(c.Y(j) - py) * 2.0 * harm;
-    v1z += (double)(c.Z(j) - pz) * 2.0 * harm;
+  nb.each([&](float qx, float qy, float qz) {
+    v1x += (double)(qx - px) * harm2;
+    v1y += (double)(qy - py) * harm2;
+    v1z += (double)(qz - pz) * harm2;
   });
   *fx = (float)((double)*fx + v1x);
   *fy = (float)((double)*fy + v1y);
   *fz = (float)((double)*fz + v1z);
 }
 
-constexpr int MAX_DEG = 8;  // bonds per bead handled by the sweep kernels
-
 // One move of mc_polymer_move_arr (device_function.cu:747-790) with
 // trial_move_smc_arr (:637-717), calc_delta_energy_arr (:562-588),
 // calc_delta_bonded_energy_arr (:509-561) and som_accept (:444-454) inlined.
 // Returns 1 if accepted.
-template <class KP, class V>
-__device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, int p, int length,
-                                             uint32_t chain, uint32_t sweep, uint32_t i,
-                                             const SCMF_GLOBAL float* __restrict__ omega) {
+template <bool LINEAR, class KP, class V>
+__device__ __forceinline__ uint32_t one_move_n(const KP& kp, const V& c, int p, int length,
+                                               uint32_t chain, uint32_t sweep, uint32_t i,
+                                               const SCMF_GLOBAL float* __restrict__ omega) {
   const u32x4 pick = stream_block(kp.seed, kp.system_id, chain, sweep, i, SLOT_PICK);
   int ibead = (int)(u01_53(pick.v[0], pick.v[1]) * length);
   if (ibead == length) ibead = length - 1;
   const uint32_t iwtype = (uint32_t)gptr((const int*)kp.monotype)[kp.mono_off[p] + ibead];
   const u32x4 disp = stream_block(kp.seed, kp.system_id, chain, sweep, i, SLOT_DISP);
   const float ox = c.X(ibead), oy = c.Y(ibead), oz = c.Z(ibead);
+  const Neighbours<LINEAR, KP, V> nb(kp, c, p, ibead, length);
+  const double harm2 = kp.harm * 2.0;
   float dx, dy, dz;
   double smc_deltaE = 0.0;
   if (kp.move_type == SCMF_MOVE_SMART) {
@@ -161,7 +200,7 @@ __device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, int p, in
     const float A = kp.A[iwtype];
     const float R = kp.R[iwtype];
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
-    bond_force(kp, c, p, ibead, length, px, py, pz, &fx, &fy, &fz);
+    bond_force(nb, harm2, px, py, pz, &fx, &fy, &fz);
     float rx, ry, rz;
     normal3f(disp, &rx, &ry, &rz);
     dx = A * fx + rx * R;
@@ -171,7 +210,7 @@ __device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, int p, in
     py += dy;
     pz += dz;
     float nfx = 0.0f, nfy = 0.0f, nfz = 0.0f;
-    bond_force(kp, c, p, ibead, length, px, py, pz, &nfx, &nfy, &nfz);
+    bond_force(nb, harm2, px, py, pz, &nfx, &nfy, &nfz);
     smc_deltaE = 0.0;
     smc_deltaE += 0.5 * ((nfx + fx) * dx + (nfy + fy) * dy + (nfz + fz) * dz);
     smc_deltaE +=
@@ -192,12 +231,12 @@ __device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, int p, in
   double bonded = 0;
   {
     const double ddx = dx, ddy = dy, ddz = dz;
-    for_neighbours(kp, p, ibead, length, [&](int j) {
-      const double old_rx = (double)(ox - c.X(j));
+    nb.each([&](float qx, float qy, float qz) {
+      const double old_rx = (double)(ox - qx);
       const double new_rx = old_rx + ddx;
-      const double old_ry = (double)(oy - c.Y(j));
+      const double old_ry = (double)(oy - qy);
       const double new_ry = old_ry + ddy;
-      const double old_rz = (double)(oz - c.Z(j));
+      const double old_rz = (double)(oz - qz);
       const double new_rz = old_rz + ddz;
       const double old_r2 = old_rx * old_rx + old_ry * old_ry + old_rz * old_rz;
       const double new_r2 = new_rx * new_rx + new_ry * new_ry + new_rz * new_rz;
@@ -214,6 +253,15 @@ __device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, int p, in
     return 1u;
   }
   return 0u;
+}
+
+// The move with neighbour storage sized for the chain's architecture.
+template <class KP, class V>
+__device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, int p, int length,
+                                             uint32_t chain, uint32_t sweep, uint32_t i,
+                                             const SCMF_GLOBAL float* __restrict__ omega) {
+  if (kp.linear[p]) return one_move_n<true>(kp, c, p, length, chain, sweep, i, omega);
+  return one_move_n<false>(kp, c, p, length, chain, sweep, i, omega);
 }
 
 __device__ __forceinline__ void wave_count(uint32_t acc, unsigned long long* accepts) {
@@ -314,8 +362,10 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
 // W[N+2] (level per bead during the scan, bead b at W[b+1] with zero
 // sentinels around, then per-level fill counts); start[N+2] (first list
 // entry of each level); list[N] (moves by level).
+// (At most 96 VGPRs, i.e. five waves per SIMD: the LDS tiles allow about
+// as many, and the move is issue-bound, so occupancy is worth the spills.)
 template <int L, typename I>
-__global__ void __launch_bounds__(64) k_sweep_sched(const KParams* __restrict__ Ps,
+__global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict__ Ps,
                                                     const BlockRef* __restrict__ bt, int nmax) {
   constexpr int G = 64 / L;
   extern __shared__ float lds[];
