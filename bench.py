@@ -233,11 +233,30 @@ def measure_system(workload, steps, warmup):
     tiled = spec.n_beads >= (1 << 23)
     sweep_kernel = {-1: "k_sweep_global", 1: "k_sweep_lds"}.get(lanes, f"k_sweep_sched<{lanes}>")
 
-    def roof(nbytes, ms):
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    # of this workload (calibration and commands in the profile summary).
+    measured = {}
+    tpath = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            measured = json.load(f).get("bytes_per_launch", {})
+    byte_t = "unsigned char" if spec.polymers and max(p.length for p in spec.polymers) <= 255 else "unsigned short"
+    sweep_key = f"k_sweep_sched<{lanes}, {byte_t}>" if lanes > 1 else sweep_kernel
+    grid_keys = [f"k_grid_tiles<{bits}, true>", f"k_tiles_sum<{bits}>", "k_zero_counts"]
+
+    def traffic_of(keys):
+        if all(k in measured for k in keys):
+            return sum(measured[k] for k in keys)
+        return None
+
+    def roof(nbytes, ms, traffic):
         gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": nbytes, "ms": round(ms, 4)}
+        out = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+               "algorithmic_bytes": nbytes, "ms": round(ms, 4)}
+        if traffic is not None:
+            out["traffic_source"] = f"profiles/{workload}_kernel_stats.md"
+        return out
 
     return {
         "_spec": spec, "_opts": opts,
@@ -251,11 +270,14 @@ def measure_system(workload, steps, warmup):
                    "box_Re": list(spec.L), "xn": spec.xn, "move": move,
                    "density_bits": bits, "omega": "every sweep", "parallelism": "1 GPU",
                    "lanes_per_chain": lanes},
-        "roofline": dict(roof(sweep_bytes, tm.sweep_ms), kernel=sweep_kernel),
-        "passes": {"grid": dict(roof(grid_bytes, tm.grid_ms),
+        "roofline": dict(roof(sweep_bytes, tm.sweep_ms, traffic_of([sweep_key])),
+                         kernel=sweep_kernel),
+        "passes": {"grid": dict(roof(grid_bytes, tm.grid_ms,
+                                     traffic_of(grid_keys) if tiled else None),
                                 kernel="k_zero_counts+k_grid_tiles+k_tiles_sum" if tiled
                                 else "k_zero_counts+k_grid"),
-                   "omega": dict(roof(omega_bytes, tm.omega_ms), kernel="k_omega")},
+                   "omega": dict(roof(omega_bytes, tm.omega_ms, traffic_of(["k_omega"])),
+                                 kernel="k_omega")},
         "acceptance": st.accepts / max(st.moves, 1),
         "grid_fallback_beads": int(st.grid_fallback),
     }
