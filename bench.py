@@ -185,7 +185,7 @@ def served(n_jobs, rate, seed=0, steps_per_tick=25):
                        "cells), chiN 10-50, 500-5000 sweeps, 128 beads/Re^3, uint16"}
 
 
-def measure_system(workload, steps, warmup):
+def measure_system(workload, steps, warmup, with_host_copy=False):
     """Create the named system, autotune the sweep shape, run `warmup` steps,
     then time `steps` steps (sweep + gridding + omega every sweep)."""
     from new_arch_amd import scmf
@@ -220,6 +220,22 @@ def measure_system(workload, steps, warmup):
     t1 = time.perf_counter()
     tm = g.timing()
     st = g.stats()
+    host = None
+    if with_host_copy:
+        # The rate a caller would see if it copied the positions out and back
+        # in every step (not the headline: the engine keeps them resident).
+        t = time.perf_counter()
+        pos = g.positions()
+        t_out = time.perf_counter() - t
+        t = time.perf_counter()
+        g.set_positions(pos)
+        g.sync()
+        t_in = time.perf_counter() - t
+        del pos
+        step_s = (t1 - t0) / steps
+        host = {"copy_out_ms": round(t_out * 1e3, 2), "copy_in_ms": round(t_in * 1e3, 2),
+                "position_bytes": 12 * spec.n_beads,
+                "bead_moves_per_s_with_copies": spec.n_beads / (step_s + t_out + t_in)}
     g.close()
     log(f"{workload}: timed {steps} steps in {t1 - t0:.3f} s; sweep {tm.sweep_ms:.3f} ms, "
         f"grid {tm.grid_ms:.3f} ms, omega {tm.omega_ms:.3f} ms; acceptance "
@@ -280,6 +296,7 @@ def measure_system(workload, steps, warmup):
                                  kernel="k_omega")},
         "acceptance": st.accepts / max(st.moves, 1),
         "grid_fallback_beads": int(st.grid_fallback),
+        "pcie_inclusive": host,
     }
 
 
@@ -301,7 +318,7 @@ def main():
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
 
-    m = measure_system(args.workload, args.steps, args.warmup)
+    m = measure_system(args.workload, args.steps, args.warmup, with_host_copy=True)
     spec, opts = m.pop("_spec"), m.pop("_opts")
     out = {
         "metric": "bead-moves/s/GPU",
@@ -320,6 +337,7 @@ def main():
         "roofline": m["roofline"],
         "passes": m["passes"],
         "acceptance": m["acceptance"],
+        "pcie_inclusive": m["pcie_inclusive"],
     }
     if not args.no_largest and args.workload != "largest_blend":
         log("largest system")
