@@ -1,0 +1,140 @@
+"""Edge cases of the HIP engine against the oracle, bit for bit: chains longer
+than 255 beads (16-bit schedule indices), one-, two- and three-bead chains,
+architectures with no chains, one and eight monomer types, tiled gridding
+with cells that are not a power of two, and a cell holding more beads than a
+16-bit counter (the window normalisation of the tiled gridding)."""
+import numpy as np
+import pytest
+
+from new_arch_amd import scmf
+from new_arch_amd import workloads as W
+from new_arch_amd.abi import PolymerType, SystemSpec, make_opts
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def banner(msg):
+    print(f"[gpu-edges] {msg}", flush=True)
+
+
+def same_bits(a, b):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    assert a.shape == b.shape and a.dtype == b.dtype
+    return np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def run_pair(spec, sweeps=2, **kw):
+    """init, grid, omega, then sweeps with a grid + omega after each: every
+    stage bit-exact against the oracle."""
+    opts = make_opts(**kw)
+    g = scmf.System(spec, opts)
+    try:
+        o = orc.Oracle(spec, opts)
+        pos = o.init_coords()
+        assert same_bits(g.positions(), pos)
+        g.grid_density()
+        c = g.density()
+        assert np.array_equal(c, o.grid(pos))
+        g.update_omega()
+        om = g.omega()
+        assert same_bits(om, o.omega(c).astype(np.float32))
+        for k in range(sweeps):
+            g.reset_stats()
+            acc = o.sweep(pos, om, k)
+            g.mc_sweep(1)
+            assert g.stats().accepts == acc
+            assert same_bits(g.positions(), pos)
+            g.grid_density()
+            c = g.density()
+            assert np.array_equal(c, o.grid(pos))
+            g.update_omega()
+            om = g.omega()
+            assert same_bits(om, o.omega(c).astype(np.float32))
+    finally:
+        g.close()
+
+
+def long_chain_spec():
+    return SystemSpec(n_types=2, xn=W.xn_matrix(2, 50.0, 20.0), L=[3.0, 3.0, 3.0], n=[12, 12, 12],
+                      dtNxi=0.17, nref=300,
+                      polymers=[W.diblock(300, 0.4, 40), W.homopolymer(20, 100, 1)])
+
+
+@pytest.mark.parametrize("lanes", [0, -1, 4, 8])
+def test_chains_longer_than_255(lanes):
+    banner(f"N=300, lanes {lanes}")
+    run_pair(long_chain_spec(), seed=11, lanes_per_chain=lanes, density_bits=32)
+
+
+def test_long_chains_reject_shapes_that_do_not_fit():
+    with pytest.raises(scmf.ScmfError):
+        scmf.System(long_chain_spec(), make_opts(seed=1, lanes_per_chain=1))
+
+
+def short_chain_spec():
+    return SystemSpec(n_types=2, xn=W.xn_matrix(2, 40.0, 25.0), L=[2.0, 2.0, 2.0], n=[8, 8, 8],
+                      dtNxi=0.3, nref=16,
+                      polymers=[W.homopolymer(1, 200, 0), W.diblock(2, 0.5, 150),
+                                PolymerType(length=3, count=100, monotype=[1, 0, 1]),
+                                W.homopolymer(7, 0, 1),  # an architecture with no chains
+                                W.diblock(16, 0.5, 50)])
+
+
+@pytest.mark.parametrize("move", ["smart", "trial"])
+@pytest.mark.parametrize("lanes", [0, -1, 1, 8])
+def test_one_two_three_bead_chains(move, lanes):
+    banner(f"short chains {move} lanes {lanes}")
+    run_pair(short_chain_spec(), seed=12, move=move, trial_delta=0.15, lanes_per_chain=lanes)
+
+
+@pytest.mark.parametrize("T", [1, 8])
+def test_one_and_eight_monomer_types(T):
+    banner(f"T={T}")
+    poly = PolymerType(length=24, count=300, monotype=[i % T for i in range(24)])
+    spec = SystemSpec(n_types=T, xn=W.xn_matrix(T, 45.0, 18.0), L=[2.5, 2.0, 3.0],
+                      n=[10, 8, 12], dtNxi=0.2, nref=24, polymers=[poly])
+    run_pair(spec, seed=13, density_bits=16)
+
+
+@pytest.mark.parametrize("bits", [16, 32])
+def test_tiled_gridding_general_cells(bits):
+    """Cells of 0.1 Re (not a power of two): the tiled gridding takes the
+    general binning path; uneven tiles (two per axis, one cell longer) with
+    halos on every axis; ordering every 2 sweeps."""
+    banner(f"tiled, 0.1 Re cells, u{bits}")
+    grid = (31, 27, 35)
+    spec = SystemSpec(n_types=2, xn=W.xn_matrix(2, 40.0, 15.0), L=[g * 0.1 for g in grid],
+                      n=list(grid), dtNxi=0.1, nref=16,
+                      polymers=[PolymerType(length=8, count=3000,
+                                            monotype=[t % 2 for t in range(8)])])
+    run_pair(spec, sweeps=4, seed=14, density_bits=bits, sort_every=2)
+
+
+@pytest.mark.parametrize("bits", [16, 32])
+def test_crowded_cell(bits):
+    """70,000 single beads in one cell: the tile holds more than 65,535 beads,
+    so its window is normalised while binning.  uint32 counts stay exact;
+    uint16 counts raise the overflow flag."""
+    banner(f"crowded cell u{bits}")
+    spec = SystemSpec(n_types=2, xn=W.xn_matrix(2, 40.0, 15.0), L=[2.0, 2.0, 2.0],
+                      n=[16, 16, 16], dtNxi=0.1, nref=16,
+                      polymers=[W.homopolymer(1, 70_000, 0), W.homopolymer(1, 5_000, 1)])
+    g = scmf.System(spec, make_opts(seed=15, density_bits=bits, sort_every=1))
+    try:
+        o = orc.Oracle(spec, g.opts)
+        pos = o.init_coords()
+        pos[:70_000] = np.array([0.51, 1.26, 0.77], np.float32)  # all type-0 beads in one cell
+        g.set_positions(pos)
+        g.grid_density()
+        want = o.grid(pos)
+        assert want.max() == 70_000
+        if bits == 32:
+            assert np.array_equal(g.density(), want)
+            assert g.stats().overflow == 0
+        else:
+            assert g.stats().overflow == 1
+        assert g.stats().sorts >= 1
+    finally:
+        g.close()
