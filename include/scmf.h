@@ -186,6 +186,16 @@ int scmf_sizes(scmf_sys* sys, int64_t* n_beads, int64_t* n_chains, int64_t* n_ce
 int scmf_write_density(scmf_sys* sys, const char* dir, int32_t step);
 /* io_out_coord (io.cpp:95-118): "%g %g %g" per bead. */
 int scmf_write_coord(scmf_sys* sys, const char* path);
+
+/* Exact checkpoint, the lossless counterpart of coord.dat restarts
+   (io.cpp:95-118 with init_config.cu:930-987, which lose digits to %g):
+   positions, counts, omega, seed, system id and the stream sweep counter, so
+   a run restored from it continues bit for bit.  `step` is stored for the
+   caller (the omega cadence of scmf_step depends on it). */
+int scmf_checkpoint_write(scmf_sys* sys, const char* path, int64_t step);
+/* Restore into a system created from the same description and counter
+   width; returns the stored step. */
+int scmf_checkpoint_read(scmf_sys* sys, const char* path, int64_t* step);
 /* Host-only helpers (no device needed). */
 /* Parse configuration.dat + polymer.dat and report sizes and memory needs. */
 int scmf_inspect_files(const char* configuration_dat, const char* polymer_dat, scmf_summary* out);

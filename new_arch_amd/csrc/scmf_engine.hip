@@ -1363,6 +1363,55 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
 }  // namespace
 
 // ============================================================== C ABI
+namespace {
+
+// Checkpoint file: header, positions (fp32 [bead][xyz], reference order),
+// counts (uint32 [type][cell]), omega (fp32 [type][cell]); little-endian.
+struct CheckpointHeader {
+  char magic[8];
+  uint32_t version, T;
+  int64_t n_beads, n_chains, n_cells;
+  int32_t n[3], density_bits;
+  uint64_t seed;
+  uint32_t system_id, pad;
+  uint64_t next_sweep;
+  int64_t step;
+  uint64_t desc_hash;
+};
+const char kCheckpointMagic[8] = {'S', 'C', 'M', 'F', 'C', 'K', 'P', 'T'};
+
+// FNV-1a over everything that defines the system.
+struct Fnv {
+  uint64_t h = 1469598103934665603ull;
+  void add(const void* p, size_t n) {
+    const uint8_t* b = (const uint8_t*)p;
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  }
+  template <class T>
+  void vec(const std::vector<T>& v) {
+    const uint64_t n = v.size();
+    add(&n, sizeof n);
+    if (n) add(v.data(), sizeof(T) * v.size());
+  }
+};
+
+uint64_t desc_hash(const HostDesc& d) {
+  Fnv f;
+  f.add(&d.T, sizeof d.T);
+  f.vec(d.xn);
+  f.add(d.L, sizeof d.L);
+  f.add(d.n, sizeof d.n);
+  f.add(&d.dtNxi, sizeof d.dtNxi);
+  f.add(&d.nref, sizeof d.nref);
+  f.vec(d.poly_length);
+  f.vec(d.n_polymers);
+  f.vec(d.monotype);
+  f.vec(d.connection);
+  return f.h;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* scmf_last_error(void) { return g_err.c_str(); }
@@ -1768,6 +1817,82 @@ int scmf_write_coord(scmf_sys* s, const char* path) {
   if (rc) return rc;
   std::string e = write_coord_file(path, xyz.data(), s->n_beads);
   if (!e.empty()) return fail(SCMF_E_IO, e);
+  return SCMF_OK;
+}
+
+int scmf_checkpoint_write(scmf_sys* s, const char* path, int64_t step) {
+  if (!s || !path) return fail(SCMF_E_ARG, "null argument");
+  const int64_t TC = (int64_t)s->T * s->n_cells;
+  CheckpointHeader h{};
+  memcpy(h.magic, kCheckpointMagic, 8);
+  h.version = 1;
+  h.T = (uint32_t)s->T;
+  h.n_beads = s->n_beads;
+  h.n_chains = s->n_chains;
+  h.n_cells = s->n_cells;
+  for (int a = 0; a < 3; ++a) h.n[a] = s->desc.n[a];
+  h.density_bits = s->opts.density_bits;
+  h.seed = s->opts.seed;
+  h.system_id = s->opts.system_id;
+  h.next_sweep = s->sweeps;
+  h.step = step;
+  h.desc_hash = desc_hash(s->desc);
+  std::vector<float> pos(3 * (size_t)s->n_beads);
+  std::vector<uint32_t> counts((size_t)TC);
+  std::vector<float> om((size_t)TC);
+  int rc;
+  if ((rc = scmf_copy_out(s, SCMF_POS, pos.data(), sizeof(float) * pos.size())) ||
+      (rc = scmf_copy_out(s, SCMF_DENSITY, counts.data(), sizeof(uint32_t) * counts.size())) ||
+      (rc = scmf_copy_out(s, SCMF_OMEGA, om.data(), sizeof(float) * om.size())))
+    return rc;
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "wb");
+  if (!f) return fail(SCMF_E_IO, "cannot open " + tmp + " for writing");
+  bool ok = fwrite(&h, sizeof h, 1, f) == 1 && fwrite(pos.data(), sizeof(float), pos.size(), f) == pos.size() &&
+            fwrite(counts.data(), sizeof(uint32_t), counts.size(), f) == counts.size() &&
+            fwrite(om.data(), sizeof(float), om.size(), f) == om.size();
+  ok = (fclose(f) == 0) && ok;
+  if (!ok || rename(tmp.c_str(), path) != 0) {
+    remove(tmp.c_str());
+    return fail(SCMF_E_IO, std::string("writing the checkpoint ") + path + " failed");
+  }
+  return SCMF_OK;
+}
+
+int scmf_checkpoint_read(scmf_sys* s, const char* path, int64_t* step) {
+  if (!s || !path) return fail(SCMF_E_ARG, "null argument");
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(SCMF_E_IO, std::string("cannot open ") + path);
+  CheckpointHeader h{};
+  const int64_t TC = (int64_t)s->T * s->n_cells;
+  auto bad = [&](const std::string& why) {
+    fclose(f);
+    return fail(SCMF_E_IO, std::string(path) + ": " + why);
+  };
+  if (fread(&h, sizeof h, 1, f) != 1) return bad("truncated header");
+  if (memcmp(h.magic, kCheckpointMagic, 8) != 0 || h.version != 1) return bad("not a checkpoint file");
+  if ((int)h.T != s->T || h.n_beads != s->n_beads || h.n_chains != s->n_chains ||
+      h.n_cells != s->n_cells || h.desc_hash != desc_hash(s->desc))
+    return bad("written for a different system description");
+  if (h.density_bits != s->opts.density_bits) return bad("written with other density counters");
+  if (h.next_sweep >= 0xFFFFFFFFull) return bad("sweep counter out of range");
+  std::vector<float> pos(3 * (size_t)s->n_beads);
+  std::vector<uint32_t> counts((size_t)TC);
+  std::vector<float> om((size_t)TC);
+  if (fread(pos.data(), sizeof(float), pos.size(), f) != pos.size() ||
+      fread(counts.data(), sizeof(uint32_t), counts.size(), f) != counts.size() ||
+      fread(om.data(), sizeof(float), om.size(), f) != om.size())
+    return bad("truncated data");
+  fclose(f);
+  int rc;
+  if ((rc = scmf_copy_in_pos(s, pos.data(), (size_t)s->n_beads)) ||
+      (rc = scmf_copy_in_density(s, counts.data(), (size_t)TC)) ||
+      (rc = scmf_copy_in_omega(s, om.data(), (size_t)TC)))
+    return rc;
+  s->opts.system_id = h.system_id;
+  s->kp.system_id = h.system_id;
+  if ((rc = scmf_set_rng(s, h.seed, h.next_sweep))) return rc;
+  if (step) *step = h.step;
   return SCMF_OK;
 }
 

@@ -8,6 +8,9 @@
 //   -seed=S            Philox key (default: time(NULL), like init_config.cu:864)
 //   -move=smart|trial  -trial_delta=D  -omega_every_step=0|1  -density_bits=16|32
 //   -config=PATH -polymer=PATH -outdir=DIR
+//   -checkpoint=PATH   exact checkpoint after the last step
+//   -restart=PATH      continue from such a checkpoint (instead of the
+//                      initial gridding; the run starts after its step)
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -71,7 +74,7 @@ int main(int argc, char** argv) {
     return 2;
   }
   const int tcheck = (int)flag_int(argc, argv, "Check_T", 1000);
-  const int start_time = (int)flag_int(argc, argv, "Start_time", 0);
+  int start_time = (int)flag_int(argc, argv, "Start_time", 0);
   const int n_steps = (int)flag_int(argc, argv, "Steps_N", 100);
   const int read_file = (int)flag_int(argc, argv, "read_file", 0);
   const char* cfg = flag_value(argc, argv, "config");
@@ -80,6 +83,8 @@ int main(int argc, char** argv) {
   const char* mv = flag_value(argc, argv, "move");
   const char* td = flag_value(argc, argv, "trial_delta");
   const char* sd = flag_value(argc, argv, "seed");
+  const char* restart = flag_value(argc, argv, "restart");        // exact checkpoint to continue
+  const char* checkpoint = flag_value(argc, argv, "checkpoint");  // written after the last step
 
   scmf_opts o{};
   o.seed = sd ? strtoull(sd, nullptr, 10) : (uint64_t)time(nullptr);
@@ -104,9 +109,18 @@ int main(int argc, char** argv) {
       printf("coord.dat did not exist or was short; keeping the generated configuration.\n");
     }
   }
-  // test.cpp:21-23
-  if ((rc = scmf_grid_density(sys))) return die("density_reset_mpi", rc);
-  if ((rc = scmf_update_omega(sys))) return die("update_omega_fields_scmf0", rc);
+  if (restart) {
+    // positions, counts, omega and the stream position of the checkpoint;
+    // the run continues after its step
+    int64_t step0 = 0;
+    if ((rc = scmf_checkpoint_read(sys, restart, &step0))) return die("restart", rc);
+    start_time = (int)step0;
+    printf("restarted from %s at step %d\n", restart, start_time);
+  } else {
+    // test.cpp:21-23
+    if ((rc = scmf_grid_density(sys))) return die("density_reset_mpi", rc);
+    if ((rc = scmf_update_omega(sys))) return die("update_omega_fields_scmf0", rc);
+  }
   // test.cpp:37-73
   for (int step = start_time + 1; step <= n_steps + start_time; step++) {
     if ((rc = scmf_step(sys, step))) return die("step", rc);
@@ -120,6 +134,8 @@ int main(int argc, char** argv) {
   }
   std::string coord = outdir ? std::string(outdir) + "/coord.dat" : std::string("coord.dat");
   if ((rc = scmf_write_coord(sys, coord.c_str()))) return die("io_out_coord", rc);
+  if (checkpoint && (rc = scmf_checkpoint_write(sys, checkpoint, n_steps + start_time)))
+    return die("checkpoint", rc);
   // test.cpp:82-95: one timed step.
   scmf_sync(sys);
   hipEvent_t a, b;

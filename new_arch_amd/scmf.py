@@ -52,6 +52,8 @@ def lib():
                              C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.scmf_write_density.argtypes = [vp, C.c_char_p, C.c_int32]
     L.scmf_write_coord.argtypes = [vp, C.c_char_p]
+    L.scmf_checkpoint_write.argtypes = [vp, C.c_char_p, C.c_int64]
+    L.scmf_checkpoint_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
     L.scmf_density_to_vtk.argtypes = [C.c_char_p, C.c_char_p]
     L.scmf_inspect_files.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Summary)]
     L.scmf_inspect_desc.argtypes = [C.POINTER(Desc), C.POINTER(Summary)]
@@ -225,6 +227,14 @@ class System:
     # -- files
     def write_density(self, step, directory=""):
         _check(lib().scmf_write_density(self._h, str(directory).encode(), int(step)))
+
+    def checkpoint(self, path, step):
+        _check(lib().scmf_checkpoint_write(self._h, str(path).encode(), int(step)))
+
+    def restore(self, path) -> int:
+        step = C.c_int64()
+        _check(lib().scmf_checkpoint_read(self._h, str(path).encode(), C.byref(step)))
+        return step.value
 
     def write_coord(self, path):
         _check(lib().scmf_write_coord(self._h, str(path).encode()))
