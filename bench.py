@@ -15,6 +15,7 @@ on a bounded sample of the same workload on this host.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -236,6 +237,24 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
         host = {"copy_out_ms": round(t_out * 1e3, 2), "copy_in_ms": round(t_in * 1e3, 2),
                 "position_bytes": 12 * spec.n_beads,
                 "bead_moves_per_s_with_copies": spec.n_beads / (step_s + t_out + t_in)}
+    analysis = None
+    if with_host_copy:
+        # The composition structure factor of the final state on the device
+        # (weights, in-place D2Z FFT, shell sums), as a run would take it every
+        # check interval; synchronous, shell sums copied to the host.
+        coef = [1.0, -1.0] + [0.0] * (spec.n_types - 2)
+        dq = 2.0 * math.pi / spec.L[0]
+        n_bins = spec.n[0]
+        for _ in range(2):
+            g.structure_factor(coef, dq, n_bins)
+        reps = 10
+        t = time.perf_counter()
+        for _ in range(reps):
+            sums, modes = g.structure_factor(coef, dq, n_bins)
+        analysis = {"structure_factor_ms": round((time.perf_counter() - t) * 1e3 / reps, 3),
+                    "grid": list(spec.n), "shells": n_bins, "modes": int(modes.sum()),
+                    "S_first_shell": float(sums[1] / max(modes[1], 1))}
+        log(f"structure factor: {analysis['structure_factor_ms']} ms per call")
     g.close()
     log(f"{workload}: timed {steps} steps in {t1 - t0:.3f} s; sweep {tm.sweep_ms:.3f} ms, "
         f"grid {tm.grid_ms:.3f} ms, omega {tm.omega_ms:.3f} ms; acceptance "
@@ -297,6 +316,7 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
         "acceptance": st.accepts / max(st.moves, 1),
         "grid_fallback_beads": int(st.grid_fallback),
         "pcie_inclusive": host,
+        "analysis": analysis,
     }
 
 
@@ -338,6 +358,7 @@ def main():
         "passes": m["passes"],
         "acceptance": m["acceptance"],
         "pcie_inclusive": m["pcie_inclusive"],
+        "analysis": m["analysis"],
     }
     if not args.no_largest and args.workload != "largest_blend":
         log("largest system")

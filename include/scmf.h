@@ -196,6 +196,19 @@ int scmf_checkpoint_write(scmf_sys* sys, const char* path, int64_t step);
 /* Restore into a system created from the same description and counter
    width; returns the stored step. */
 int scmf_checkpoint_read(scmf_sys* sys, const char* path, int64_t* step);
+/* Static structure factor of the current counts, an analysis the reference
+   names only as a link dependency (-lcufft, Makefile:265):
+   w(c) = sum_t coef[t] * n_t(c), F(q) = sum_c w(c) exp(-i q.r_c) over the
+   grid's wave vectors q = 2*pi*(kx/Lx, ky/Ly, kz/Lz), S(q) = |F(q)|^2 / n_beads.
+   Each mode goes to shell b = nearest integer to |q|/dq (by_q2 = 0) or to
+   q^2/dq^2 (by_q2 = 1; with dq = 2*pi/L on a cubic box the shells are the
+   integer |k|^2); shells b >= n_bins are dropped.  On return s_sum[b] is the
+   sum of S over the modes of shell b (q = 0 is in shell 0) and n_modes[b]
+   their number (every mode of the full spectrum counts once).  Sums are fp64
+   atomics: reproducible to rounding, not bit for bit.  Synchronous. */
+#define SCMF_SQ_MAX_BINS (1 << 20)
+int scmf_structure_factor(scmf_sys* sys, const double* coef, double dq, int32_t by_q2,
+                          int32_t n_bins, double* s_sum, int64_t* n_modes);
 /* Host-only helpers (no device needed). */
 /* Parse configuration.dat + polymer.dat and report sizes and memory needs. */
 int scmf_inspect_files(const char* configuration_dat, const char* polymer_dat, scmf_summary* out);

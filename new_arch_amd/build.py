@@ -34,7 +34,8 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, "-shared", "-o", tmp,
           os.path.join(CSRC, "scmf_engine.hip"), os.path.join(CSRC, "scmf_host.cpp"),
-          os.path.join(CSRC, "scmf_server.cpp")])
+          os.path.join(CSRC, "scmf_server.cpp"), "-L/opt/rocm/lib", "-lhipfft",
+          "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     tmp = RUNNER + ".tmp"
     _run([HIPCC, *COMMON, "-o", tmp, os.path.join(CSRC, "scmf_main.cpp"),

@@ -15,10 +15,12 @@
 //   k_grid         one lane per bead: NGP cell, atomic increment
 //   k_omega        one lane per cell: all T omega values in fp64, stored fp32
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 #include <stdio.h>
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -592,6 +594,89 @@ __global__ void __launch_bounds__(BLOCK) k_re2(KParams kp, const float* __restri
   if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
+// ------------------------------------------------ structure factor
+// S(q) = |sum_c w(c) exp(-i q.r_c)|^2 / n_beads with w(c) = sum_t coef[t]
+// n_t(c), from an in-place double-precision real-to-complex FFT.
+
+struct SqCoef {
+  double c[MAX_T];
+};
+
+// w(c) into the padded rows of the in-place transform: row (z, y) holds
+// 2*(nx/2+1) doubles, the first nx of them the cells x = 0..nx-1.
+__global__ void __launch_bounds__(BLOCK) k_sq_weights(KParams kp, SqCoef coef,
+                                                      double* __restrict__ w, int64_t row_pad) {
+  const uint32_t* __restrict__ words = kp.counts;
+  const uint32_t C = kp.n_cells;
+  const bool b32 = kp.density_bits == 32;
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t c = (int64_t)blockIdx.x * BLOCK + threadIdx.x; c < C; c += stride) {
+    double v = 0;
+    for (int t = 0; t < kp.T; ++t) {
+      const uint32_t idx = (uint32_t)c + (uint32_t)t * C;
+      const uint32_t n = b32 ? words[idx] : (words[idx >> 1] >> ((idx & 1) * 16)) & 0xFFFFu;
+      v += coef.c[t] * (double)n;
+    }
+    const int64_t row = c / kp.nx;
+    w[row * row_pad + (c - row * kp.nx)] = v;
+  }
+}
+
+constexpr int SQ_LDS_BINS = 2048;
+
+// One lane per stored mode (kz, ky, kx) with 0 <= kx <= nx/2.  Modes with
+// 0 < kx < nx/2 (or kx = (nx-1)/2 for odd nx) stand for themselves and
+// their conjugate partner at -k, so they count twice.  The shell of a mode
+// is the nearest integer to |q|/dq, or to q^2/dq^2 when by_q2 is set.
+__global__ void __launch_bounds__(BLOCK) k_sq_shells(const double* __restrict__ F, int nx, int ny,
+                                                     int nz, double gx, double gy, double gz,
+                                                     double inv_dq, int by_q2, double inv_n,
+                                                     int n_bins, double* __restrict__ sum,
+                                                     unsigned long long* __restrict__ cnt) {
+  __shared__ double lsum[SQ_LDS_BINS];
+  __shared__ unsigned lcnt[SQ_LDS_BINS];
+  const int lbins = n_bins < SQ_LDS_BINS ? n_bins : SQ_LDS_BINS;
+  for (int b = threadIdx.x; b < lbins; b += BLOCK) {
+    lsum[b] = 0;
+    lcnt[b] = 0;
+  }
+  __syncthreads();
+  const int hx = nx / 2 + 1;
+  const int64_t modes = (int64_t)nz * ny * hx;
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t m = (int64_t)blockIdx.x * BLOCK + threadIdx.x; m < modes; m += stride) {
+    const int64_t r = m / hx;
+    const int kx = (int)(m - r * hx);
+    const int iy = (int)(r % ny);
+    const int iz = (int)(r / ny);
+    const int ky = 2 * iy > ny ? iy - ny : iy;
+    const int kz = 2 * iz > nz ? iz - nz : iz;
+    const double qx = gx * kx, qy = gy * ky, qz = gz * kz;
+    const double q2 = qx * qx + qy * qy + qz * qz;
+    const double u = by_q2 ? q2 * inv_dq * inv_dq : sqrt(q2) * inv_dq;
+    const double ub = rint(u);
+    if (!(ub < (double)n_bins)) continue;
+    const int b = (int)ub;
+    const double re = F[2 * m], im = F[2 * m + 1];
+    const unsigned wt = (kx == 0 || 2 * kx == nx) ? 1u : 2u;
+    const double v = (double)wt * ((re * re + im * im) * inv_n);
+    if (b < SQ_LDS_BINS) {
+      atomicAdd(&lsum[b], v);
+      atomicAdd(&lcnt[b], wt);
+    } else {
+      atomicAdd(&sum[b], v);
+      atomicAdd(&cnt[b], (unsigned long long)wt);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < lbins; b += BLOCK) {
+    if (lcnt[b]) {
+      atomicAdd(&sum[b], lsum[b]);
+      atomicAdd(&cnt[b], (unsigned long long)lcnt[b]);
+    }
+  }
+}
+
 // Stored index of reference bead r (reference order: chains by id, beads in
 // order): the chain's slot replaces its id inside its architecture block.
 __device__ __forceinline__ int64_t stored_index(const KParams& kp, int64_t r) {
@@ -769,6 +854,13 @@ struct scmf_sys {
   double* d_partial = nullptr;
   float* staging = nullptr;
   int64_t staging_beads = 0;
+  // structure factor: in-place D2Z plan and buffer, shell accumulators
+  hipfftHandle fft = 0;
+  bool fft_ready = false;
+  double* fft_buf = nullptr;
+  double* d_sq_sum = nullptr;
+  unsigned long long* d_sq_cnt = nullptr;
+  int32_t sq_cap = 0;
   uint64_t sweeps = 0;
   uint64_t moves = 0;
   bool timing = false;
@@ -1231,8 +1323,10 @@ int free_all(scmf_sys* s) {
                   (void*)s->slot_chain2, (void*)s->btype, (void*)s->keys, (void*)s->tcount,
                   (void*)s->tstart, (void*)s->tfill, (void*)s->new_slot, (void*)s->x2,
                   (void*)s->y2, (void*)s->z2, (void*)s->scratch, (void*)s->d_fallback,
-                  (void*)s->d_kp, (void*)s->d_ctr})
+                  (void*)s->d_kp, (void*)s->d_ctr, (void*)s->fft_buf, (void*)s->d_sq_sum,
+                  (void*)s->d_sq_cnt})
     if (p) (void)hipFree(p);
+  if (s->fft_ready) (void)hipfftDestroy(s->fft);
   for (Timed& t : s->pending) {
     (void)hipEventDestroy(t.a);
     (void)hipEventDestroy(t.b);
@@ -1639,6 +1733,76 @@ int scmf_set_rng(scmf_sys* s, uint64_t seed, uint64_t next_sweep) {
   int rc = sync_kp(s);
   if (rc) return rc;
   return set_ctr(s, (uint32_t)next_sweep);
+}
+
+int scmf_structure_factor(scmf_sys* s, const double* coef, double dq, int32_t by_q2,
+                          int32_t n_bins, double* s_sum, int64_t* n_modes) {
+  if (!s || !coef || !s_sum || !n_modes) return fail(SCMF_E_ARG, "null argument");
+  if (!(dq > 0) || !std::isfinite(dq)) return fail(SCMF_E_ARG, "dq must be positive and finite");
+  if (n_bins < 1 || n_bins > SCMF_SQ_MAX_BINS) return fail(SCMF_E_ARG, "n_bins out of range");
+  if (by_q2 != 0 && by_q2 != 1) return fail(SCMF_E_ARG, "by_q2 must be 0 or 1");
+  SqCoef cf{};
+  for (int t = 0; t < s->T; ++t) {
+    if (!std::isfinite(coef[t])) return fail(SCMF_E_ARG, "coefficients must be finite");
+    cf.c[t] = coef[t];
+  }
+  const int nx = s->kp.nx, ny = s->kp.ny, nz = s->kp.nz;
+  const int64_t hx = nx / 2 + 1;
+  if (!s->fft_ready) {
+    const size_t bytes = (size_t)nz * ny * hx * 2 * sizeof(double);
+    HIP_TRY(hipMalloc(&s->fft_buf, bytes));
+    hipfftHandle h = 0;
+    if (hipfftCreate(&h) != HIPFFT_SUCCESS) return fail(SCMF_E_HIP, "hipfftCreate failed");
+    long long n[3] = {nz, ny, nx};
+    long long inembed[3] = {nz, ny, 2 * hx};
+    long long onembed[3] = {nz, ny, hx};
+    size_t work = 0;
+    hipfftResult r = hipfftMakePlanMany64(h, 3, n, inembed, 1, (long long)nz * ny * 2 * hx,
+                                          onembed, 1, (long long)nz * ny * hx, HIPFFT_D2Z, 1,
+                                          &work);
+    if (r == HIPFFT_SUCCESS) r = hipfftSetStream(h, s->stream);
+    if (r != HIPFFT_SUCCESS) {
+      (void)hipfftDestroy(h);
+      return fail(SCMF_E_HIP, "hipfft plan for the structure factor failed (" +
+                                  std::to_string((int)r) + ")");
+    }
+    s->fft = h;
+    s->fft_ready = true;
+  }
+  if (s->sq_cap < n_bins) {
+    if (s->d_sq_sum) HIP_TRY(hipFree(s->d_sq_sum));
+    if (s->d_sq_cnt) HIP_TRY(hipFree(s->d_sq_cnt));
+    s->d_sq_sum = nullptr;
+    s->d_sq_cnt = nullptr;
+    s->sq_cap = 0;
+    HIP_TRY(hipMalloc(&s->d_sq_sum, (size_t)n_bins * sizeof(double)));
+    HIP_TRY(hipMalloc(&s->d_sq_cnt, (size_t)n_bins * sizeof(unsigned long long)));
+    s->sq_cap = n_bins;
+  }
+  hipLaunchKernelGGL(k_sq_weights, dim3(grid_strided(s->n_cells)), dim3(BLOCK), 0, s->stream,
+                     s->kp, cf, s->fft_buf, 2 * hx);
+  HIP_TRY(hipGetLastError());
+  const hipfftResult r =
+      hipfftExecD2Z(s->fft, (hipfftDoubleReal*)s->fft_buf, (hipfftDoubleComplex*)s->fft_buf);
+  if (r != HIPFFT_SUCCESS)
+    return fail(SCMF_E_HIP, "hipfftExecD2Z failed (" + std::to_string((int)r) + ")");
+  HIP_TRY(hipMemsetAsync(s->d_sq_sum, 0, (size_t)n_bins * sizeof(double), s->stream));
+  HIP_TRY(hipMemsetAsync(s->d_sq_cnt, 0, (size_t)n_bins * sizeof(unsigned long long), s->stream));
+  const double two_pi = 6.283185307179586;
+  const int64_t modes = (int64_t)nz * ny * hx;
+  int64_t blocks = (modes + BLOCK - 1) / BLOCK;
+  if (blocks > 1024) blocks = 1024;
+  hipLaunchKernelGGL(k_sq_shells, dim3((unsigned)blocks), dim3(BLOCK), 0, s->stream,
+                     (const double*)s->fft_buf, nx, ny, nz, two_pi / s->kp.Lx, two_pi / s->kp.Ly,
+                     two_pi / s->kp.Lz, 1.0 / dq, by_q2, 1.0 / (double)s->n_beads, n_bins,
+                     s->d_sq_sum, s->d_sq_cnt);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(s_sum, s->d_sq_sum, (size_t)n_bins * sizeof(double),
+                         hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipMemcpyAsync(n_modes, s->d_sq_cnt, (size_t)n_bins * sizeof(int64_t),
+                         hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return SCMF_OK;
 }
 
 int scmf_stats(scmf_sys* s, scmf_stats_t* out) {

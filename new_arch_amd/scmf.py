@@ -54,6 +54,7 @@ def lib():
     L.scmf_write_coord.argtypes = [vp, C.c_char_p]
     L.scmf_checkpoint_write.argtypes = [vp, C.c_char_p, C.c_int64]
     L.scmf_checkpoint_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
+    L.scmf_structure_factor.argtypes = [vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp]
     L.scmf_density_to_vtk.argtypes = [C.c_char_p, C.c_char_p]
     L.scmf_inspect_files.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Summary)]
     L.scmf_inspect_desc.argtypes = [C.POINTER(Desc), C.POINTER(Summary)]
@@ -223,6 +224,20 @@ class System:
 
     def reset_timing(self):
         _check(lib().scmf_reset_timing(self._h))
+
+    # -- analysis
+    def structure_factor(self, coef, dq, n_bins, by_q2=False):
+        """Shell sums of S(q) = |F[sum_t coef[t] n_t](q)|^2 / n_beads and the
+        number of modes per shell (shell = nearest integer to |q|/dq, or to
+        q^2/dq^2 with by_q2).  Returns (sums, n_modes)."""
+        cf = np.ascontiguousarray(coef, np.float64)
+        if cf.shape != (self.n_types,):
+            raise ValueError(f"need {self.n_types} coefficients")
+        sums = np.empty(int(n_bins), np.float64)
+        modes = np.empty(int(n_bins), np.int64)
+        _check(lib().scmf_structure_factor(self._h, cf.ctypes.data, float(dq), int(bool(by_q2)),
+                                           int(n_bins), sums.ctypes.data, modes.ctypes.data))
+        return sums, modes
 
     # -- files
     def write_density(self, step, directory=""):

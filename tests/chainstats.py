@@ -48,6 +48,14 @@ def structure_factor(counts, grid):
     return np.array([s[band].mean()] + [s[k2 == sh].mean() for sh in SHELLS])
 
 
+def structure_factor_from_shells(sums, modes):
+    """The vector of structure_factor from the shell sums and mode counts of
+    the engine's scmf_structure_factor (integer |k|^2 shells)."""
+    sh = np.asarray(SHELLS)
+    band = sums[sh[0]:sh[-1] + 1].sum() / modes[sh[0]:sh[-1] + 1].sum()
+    return np.array([band] + [sums[k] / modes[k] for k in sh])
+
+
 class Sampler:
     """Accumulates the observables of one run."""
 
@@ -64,6 +72,10 @@ class Sampler:
     def add_sample(self, re2, counts, grid):
         self.re2.append(re2)
         self.sq.append(structure_factor(counts, grid))
+
+    def add_sample_sq(self, re2, sq):
+        self.re2.append(re2)
+        self.sq.append(np.asarray(sq, np.float64))
 
     def result(self):
         return np.array([self.accepts / self.moves, np.mean(self.re2), *np.mean(self.sq, axis=0)])

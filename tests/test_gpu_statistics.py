@@ -4,8 +4,9 @@ The engine (Philox streams, seeds 1-8) runs the reference step loop on a
 110,592-bead symmetric diblock melt (N=32, chiN=20, kappaN=50, 24^3 cells,
 omega every sweep) for 200 sweeps; the oracle runs the same loop with moves
 driven by an unrelated generator (numpy PCG64, seeds 101-108).  Acceptance
-rate, <Re^2> and the low-k composition structure factor, sampled over the
-second half, agree within two standard errors of the difference of the
+rate, <Re^2> (the engine's scmf_stats) and the low-k composition structure
+factor (the engine's scmf_structure_factor on the device, numpy's FFT on
+the oracle side), sampled over the second half, agree within two standard errors of the difference of the
 means.  (With identical streams the engine equals the oracle bit for bit;
 that is tested elsewhere.)
 """
@@ -32,7 +33,11 @@ def engine_run(spec, seed, move):
                 g.reset_stats()
             g.step(sw)
             if sw >= SWEEPS // 2 and (sw + 1) % EVERY == 0:
-                smp.add_sample(g.stats().re2, g.density(), GRID)
+                sums, modes = g.structure_factor((1.0, -1.0), 2.0 * np.pi / spec.L[0],
+                                                 CS.SHELLS[-1] + 1, by_q2=True)
+                sq = CS.structure_factor_from_shells(sums, modes)
+                np.testing.assert_allclose(sq, CS.structure_factor(g.density(), GRID), rtol=1e-9)
+                smp.add_sample_sq(g.stats().re2, sq)
         st = g.stats()
         smp.add_sweep(st.accepts, st.moves)
         assert st.moves == (SWEEPS - SWEEPS // 2) * g.n_beads
