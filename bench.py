@@ -255,6 +255,27 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
                     "grid": list(spec.n), "shells": n_bins, "modes": int(modes.sum()),
                     "S_first_shell": float(sums[1] / max(modes[1], 1))}
         log(f"structure factor: {analysis['structure_factor_ms']} ms per call")
+        # A density_<step>.dat dump of this state: synchronous, and queued
+        # (the call's own cost, then the time until the file is written).
+        import shutil
+        import tempfile
+        d = tempfile.mkdtemp(prefix="scmf_dump_")
+        try:
+            t = time.perf_counter()
+            g.write_density(1, d)
+            t_sync = time.perf_counter() - t
+            size = os.path.getsize(os.path.join(d, "density_1.dat"))
+            t = time.perf_counter()
+            g.write_density_async(2, d)
+            t_call = time.perf_counter() - t
+            g.output_wait()
+            t_done = time.perf_counter() - t
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        analysis["density_dump"] = {"bytes": size, "sync_ms": round(t_sync * 1e3, 1),
+                                    "async_call_ms": round(t_call * 1e3, 3),
+                                    "async_written_ms": round(t_done * 1e3, 1)}
+        log(f"density dump: {analysis['density_dump']}")
     g.close()
     log(f"{workload}: timed {steps} steps in {t1 - t0:.3f} s; sweep {tm.sweep_ms:.3f} ms, "
         f"grid {tm.grid_ms:.3f} ms, omega {tm.omega_ms:.3f} ms; acceptance "

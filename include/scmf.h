@@ -184,6 +184,17 @@ int scmf_sizes(scmf_sys* sys, int64_t* n_beads, int64_t* n_chains, int64_t* n_ce
 
 /* io_out_density (io.cpp:4-56): density_<step>.dat in directory dir ("" = cwd). */
 int scmf_write_density(scmf_sys* sys, const char* dir, int32_t step);
+/* io_out_density without the stall: returns once a device snapshot of the
+   counts and omega is queued on the system's stream (two device-to-device
+   copies); a host thread copies it out and writes the same file as
+   scmf_write_density while the caller keeps stepping.  At most two snapshots
+   are in flight: a third call waits until the oldest file is written.  A
+   write error is reported by the next call or by scmf_output_wait.  No
+   counterpart in the reference, whose dumps are synchronous. */
+int scmf_write_density_async(scmf_sys* sys, const char* dir, int32_t step);
+/* Wait until every queued file is written; returns the first error since the
+   last report.  scmf_destroy also finishes the queue. */
+int scmf_output_wait(scmf_sys* sys);
 /* io_out_coord (io.cpp:95-118): "%g %g %g" per bead. */
 int scmf_write_coord(scmf_sys* sys, const char* path);
 

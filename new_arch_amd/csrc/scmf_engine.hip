@@ -21,7 +21,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/scmf.h"
@@ -807,6 +811,8 @@ unsigned grid_strided(int64_t n) {
 
 }  // namespace
 
+struct AsyncOut;
+
 struct scmf_sys {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -861,6 +867,7 @@ struct scmf_sys {
   double* d_sq_sum = nullptr;
   unsigned long long* d_sq_cnt = nullptr;
   int32_t sq_cap = 0;
+  AsyncOut* out = nullptr;  // asynchronous density writer, made on first use
   uint64_t sweeps = 0;
   uint64_t moves = 0;
   bool timing = false;
@@ -1506,6 +1513,147 @@ uint64_t desc_hash(const HostDesc& d) {
 
 }  // namespace
 
+// ------------------------------------------------ asynchronous output
+// Two snapshot slots.  A call takes a free slot (waiting for the oldest file
+// if both are taken), copies counts and omega into the slot's device buffers
+// on the system's stream and records an event; the writer thread waits for
+// the event, copies the slot out to pinned host memory on its own stream and
+// writes the file, then frees the slot.  The step loop only pays for the two
+// device-to-device copies.
+struct AsyncOut {
+  static constexpr int SLOTS = 2;
+  struct Slot {
+    uint32_t* d_counts = nullptr;
+    float* d_omega = nullptr;
+    uint32_t* h_counts = nullptr;  // pinned
+    float* h_omega = nullptr;      // pinned
+    hipEvent_t ready = nullptr;
+    bool busy = false;
+  };
+  struct Job {
+    int slot;
+    std::string path;
+  };
+  Slot slot[SLOTS];
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::thread th;
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Job> q;
+  bool stop = false;
+  std::string error;  // first failure since the last report
+  // what a snapshot holds
+  int64_t count_words = 0, TC = 0, n_cells = 0;
+  int T = 0, bits = 32;
+  int32_t n[3] = {0, 0, 0};
+  double L[3] = {0, 0, 0};
+  double fs = 1;
+
+  std::string write_one(const Job& j) {
+    Slot& sl = slot[j.slot];
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipEventSynchronize(sl.ready);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sl.h_counts, sl.d_counts, sizeof(uint32_t) * count_words,
+                         hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sl.h_omega, sl.d_omega, sizeof(float) * TC, hipMemcpyDeviceToHost, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return std::string("asynchronous output: ") + hipGetErrorString(e);
+    const uint32_t* counts = sl.h_counts;
+    std::vector<uint32_t> wide;
+    if (bits == 16) {
+      wide.resize((size_t)TC);
+      for (int64_t i = 0; i < TC; ++i) wide[i] = (sl.h_counts[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
+      counts = wide.data();
+    }
+    return write_density_file(j.path.c_str(), n, L, T, n_cells, counts, sl.h_omega, fs);
+  }
+
+  void run() {
+    for (;;) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || !q.empty(); });
+        if (q.empty()) return;
+        j = q.front();
+        q.pop_front();
+      }
+      std::string e = write_one(j);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (!e.empty() && error.empty()) error = e;
+        slot[j.slot].busy = false;
+      }
+      cv.notify_all();
+    }
+  }
+
+  bool idle() const {
+    if (!q.empty()) return false;
+    for (const Slot& sl : slot)
+      if (sl.busy) return false;
+    return true;
+  }
+
+  std::string take_error() {
+    std::string e;
+    e.swap(error);
+    return e;
+  }
+
+  // Drain the queue and stop the thread; frees everything.
+  void shutdown() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    if (th.joinable()) th.join();
+    for (Slot& sl : slot) {
+      if (sl.d_counts) (void)hipFree(sl.d_counts);
+      if (sl.d_omega) (void)hipFree(sl.d_omega);
+      if (sl.h_counts) (void)hipHostFree(sl.h_counts);
+      if (sl.h_omega) (void)hipHostFree(sl.h_omega);
+      if (sl.ready) (void)hipEventDestroy(sl.ready);
+    }
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+static int make_async_out(scmf_sys* s) {
+  AsyncOut* o = new AsyncOut();
+  o->device = s->device;
+  o->count_words = s->count_words;
+  o->TC = (int64_t)s->T * s->n_cells;
+  o->n_cells = s->n_cells;
+  o->T = s->T;
+  o->bits = s->opts.density_bits;
+  for (int a = 0; a < 3; ++a) {
+    o->n[a] = s->desc.n[a];
+    o->L[a] = s->desc.L[a];
+  }
+  o->fs = s->kp.fst[0];
+  hipError_t e = hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking);
+  for (AsyncOut::Slot& sl : o->slot) {
+    if (e == hipSuccess) e = hipMalloc(&sl.d_counts, sizeof(uint32_t) * o->count_words);
+    if (e == hipSuccess) e = hipMalloc(&sl.d_omega, sizeof(float) * o->TC);
+    if (e == hipSuccess) e = hipHostMalloc(&sl.h_counts, sizeof(uint32_t) * o->count_words, 0);
+    if (e == hipSuccess) e = hipHostMalloc(&sl.h_omega, sizeof(float) * o->TC, 0);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.ready, hipEventDisableTiming);
+  }
+  if (e != hipSuccess) {
+    o->shutdown();
+    delete o;
+    return fail(SCMF_E_HIP, std::string("asynchronous output buffers: ") + hipGetErrorString(e));
+  }
+  o->th = std::thread([o] { o->run(); });
+  s->out = o;
+  return SCMF_OK;
+}
+
 extern "C" {
 
 const char* scmf_last_error(void) { return g_err.c_str(); }
@@ -1549,6 +1697,11 @@ int scmf_create_from_files(scmf_sys** out, const char* configuration_dat, const 
 
 void scmf_destroy(scmf_sys* s) {
   if (!s) return;
+  if (s->out) {  // queued files are still written
+    s->out->shutdown();
+    delete s->out;
+    s->out = nullptr;
+  }
   (void)hipStreamSynchronize(s->stream);
   free_all(s);
   delete s;
@@ -1955,6 +2108,62 @@ int scmf_sizes(scmf_sys* s, int64_t* n_beads, int64_t* n_chains, int64_t* n_cell
   if (n_chains) *n_chains = s->n_chains;
   if (n_cells) *n_cells = s->n_cells;
   if (n_types) *n_types = s->T;
+  return SCMF_OK;
+}
+
+int scmf_write_density_async(scmf_sys* s, const char* dir, int32_t step) {
+  if (!s) return fail(SCMF_E_ARG, "null handle");
+  if (!s->out) {
+    int rc = make_async_out(s);
+    if (rc) return rc;
+  }
+  AsyncOut* o = s->out;
+  std::string path = density_file_name(step);
+  if (dir && *dir) path = std::string(dir) + "/" + path;
+  int k = -1;
+  {
+    std::unique_lock<std::mutex> lk(o->mu);
+    o->cv.wait(lk, [&] {
+      for (const AsyncOut::Slot& sl : o->slot)
+        if (!sl.busy) return true;
+      return false;
+    });
+    const std::string e = o->take_error();
+    if (!e.empty()) return fail(SCMF_E_IO, e);
+    for (int i = 0; i < AsyncOut::SLOTS; ++i)
+      if (!o->slot[i].busy) {
+        k = i;
+        break;
+      }
+    o->slot[k].busy = true;
+  }
+  AsyncOut::Slot& sl = o->slot[k];
+  hipError_t e = hipMemcpyAsync(sl.d_counts, s->counts, sizeof(uint32_t) * s->count_words,
+                                hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(sl.d_omega, s->omega, sizeof(float) * o->TC, hipMemcpyDeviceToDevice,
+                       s->stream);
+  if (e == hipSuccess) e = hipEventRecord(sl.ready, s->stream);
+  {
+    std::lock_guard<std::mutex> lk(o->mu);
+    if (e == hipSuccess)
+      o->q.push_back({k, path});
+    else
+      sl.busy = false;
+  }
+  o->cv.notify_all();
+  HIP_TRY(e);
+  return SCMF_OK;
+}
+
+int scmf_output_wait(scmf_sys* s) {
+  if (!s) return fail(SCMF_E_ARG, "null handle");
+  if (!s->out) return SCMF_OK;
+  AsyncOut* o = s->out;
+  std::unique_lock<std::mutex> lk(o->mu);
+  o->cv.wait(lk, [&] { return o->idle(); });
+  const std::string e = o->take_error();
+  if (!e.empty()) return fail(SCMF_E_IO, e);
   return SCMF_OK;
 }
 

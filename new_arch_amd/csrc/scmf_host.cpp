@@ -11,7 +11,10 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <memory>
+#include <thread>
+#include <vector>
 
 namespace scmf {
 
@@ -229,23 +232,87 @@ std::string density_file_name(int32_t step) {
   return name;
 }
 
+namespace {
+
+// "%g " of a double, memoised by bit pattern in a direct-mapped table: the
+// values of a density file repeat (counts are small integers, omega a
+// function of them), and glibc's %g costs ~250 ns.  A hit returns the text
+// snprintf produced for the same bits, so the output is unchanged.
+struct GFormat {
+  static constexpr int BITS = 13;
+  uint64_t key[1 << BITS];
+  uint8_t len[1 << BITS];  // 0: empty
+  char text[1 << BITS][31];
+  GFormat() { memset(len, 0, sizeof len); }
+  void append(std::string& out, double v) {
+    uint64_t b;
+    memcpy(&b, &v, 8);
+    const uint32_t h = (uint32_t)((b * 0x9E3779B97F4A7C15ull) >> (64 - BITS));
+    if (len[h] == 0 || key[h] != b) {
+      const int k = snprintf(text[h], sizeof text[h], "%g ", v);
+      key[h] = b;
+      len[h] = (uint8_t)k;
+    }
+    out.append(text[h], len[h]);
+  }
+};
+
+// Rows [c0, c1) of a density file: T scaled counts, then T omega values,
+// each printed "%g ".
+void format_density_rows(std::string& out, GFormat& g, int64_t c0, int64_t c1, int T,
+                         int64_t n_cells, const uint32_t* counts, const float* omega,
+                         double field_scaling) {
+  out.clear();
+  for (int64_t c = c0; c < c1; ++c) {
+    for (int t = 0; t < T; ++t) g.append(out, counts[c + t * n_cells] * field_scaling);
+    for (int t = 0; t < T; ++t) g.append(out, (double)omega[c + t * n_cells]);
+    out.push_back('\n');
+  }
+}
+
+}  // namespace
+
+// io_out_density's text format (io.cpp:4-56), formatted by up to eight
+// threads (each with its own GFormat) in blocks of 65,536 cells and written in cell order to PATH.tmp,
+// then renamed to PATH (a reader never sees a partial file).
 std::string write_density_file(const char* path, const int32_t n[3], const double L[3], int T,
                                int64_t n_cells, const uint32_t* counts, const float* omega,
                                double field_scaling) {
-  FILE* f = fopen(path, "w");
-  if (!f) return std::string("cannot open ") + path + " for writing";
-  std::unique_ptr<char[]> big(new char[1 << 20]);
-  setvbuf(f, big.get(), _IOFBF, 1 << 20);
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "w");
+  if (!f) return std::string("cannot open ") + tmp + " for writing";
   fprintf(f, "Nx=%d, Ny=%d, Nz=%d\n", n[0], n[1], n[2]);
   fprintf(f, "dx=%g, dy=%g, dz=%g\n", L[0] / n[0], L[1] / n[1], L[2] / n[2]);
-  for (int64_t c = 0; c < n_cells; ++c) {
-    for (int t = 0; t < T; ++t) fprintf(f, "%g ", counts[c + t * n_cells] * field_scaling);
-    for (int t = 0; t < T; ++t) fprintf(f, "%g ", (double)omega[c + t * n_cells]);
-    fputc('\n', f);
+  constexpr int64_t ROWS = 1 << 16;
+  const int64_t blocks = (n_cells + ROWS - 1) / ROWS;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int W = (int)std::min<int64_t>(std::min(8u, hw), std::max<int64_t>(blocks, 1));
+  std::vector<std::string> part((size_t)W);
+  std::vector<std::unique_ptr<GFormat>> cache((size_t)W);
+  for (auto& g : cache) g.reset(new GFormat());
+  for (int64_t b0 = 0; b0 < blocks; b0 += W) {
+    const int nb = (int)std::min<int64_t>(W, blocks - b0);
+    auto fmt = [&](int w) {
+      const int64_t c0 = (b0 + w) * ROWS;
+      format_density_rows(part[(size_t)w], *cache[(size_t)w], c0, std::min(c0 + ROWS, n_cells), T,
+                          n_cells, counts, omega, field_scaling);
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nb; ++w) th.emplace_back(fmt, w);
+    fmt(0);
+    for (auto& t : th) t.join();
+    for (int w = 0; w < nb; ++w) fwrite(part[(size_t)w].data(), 1, part[(size_t)w].size(), f);
   }
   const bool bad = ferror(f) != 0;
-  fclose(f);
-  return bad ? std::string("write error on ") + path : "";
+  if (fclose(f) != 0 || bad) {
+    remove(tmp.c_str());
+    return std::string("write error on ") + tmp;
+  }
+  if (rename(tmp.c_str(), path) != 0) {
+    remove(tmp.c_str());
+    return std::string("cannot rename ") + tmp + " to " + path;
+  }
+  return "";
 }
 
 std::string write_coord_file(const char* path, const float* xyz, int64_t n_beads) {

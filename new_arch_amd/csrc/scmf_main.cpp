@@ -11,6 +11,8 @@
 //   -checkpoint=PATH   exact checkpoint after the last step
 //   -restart=PATH      continue from such a checkpoint (instead of the
 //                      initial gridding; the run starts after its step)
+// density_<step>.dat files are written by a background thread from device
+// snapshots while the steps go on; the driver waits for them after the loop.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -129,9 +131,11 @@ int main(int argc, char** argv) {
       if ((rc = scmf_stats(sys, &st))) return die("stats", rc);
       printf("step %d end to end dis %g\n", step, st.re2);
       if (st.overflow) fprintf(stderr, "scmf_run: warning: uint16 density count overflow\n");
-      if ((rc = scmf_write_density(sys, outdir ? outdir : "", step))) return die("io_out_density", rc);
+      if ((rc = scmf_write_density_async(sys, outdir ? outdir : "", step)))
+        return die("io_out_density", rc);
     }
   }
+  if ((rc = scmf_output_wait(sys))) return die("io_out_density", rc);
   std::string coord = outdir ? std::string(outdir) + "/coord.dat" : std::string("coord.dat");
   if ((rc = scmf_write_coord(sys, coord.c_str()))) return die("io_out_coord", rc);
   if (checkpoint && (rc = scmf_checkpoint_write(sys, checkpoint, n_steps + start_time)))

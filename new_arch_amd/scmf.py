@@ -52,6 +52,8 @@ def lib():
                              C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
     L.scmf_write_density.argtypes = [vp, C.c_char_p, C.c_int32]
     L.scmf_write_coord.argtypes = [vp, C.c_char_p]
+    L.scmf_write_density_async.argtypes = [vp, C.c_char_p, C.c_int32]
+    L.scmf_output_wait.argtypes = [vp]
     L.scmf_checkpoint_write.argtypes = [vp, C.c_char_p, C.c_int64]
     L.scmf_checkpoint_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
     L.scmf_structure_factor.argtypes = [vp, vp, C.c_double, C.c_int32, C.c_int32, vp, vp]
@@ -242,6 +244,14 @@ class System:
     # -- files
     def write_density(self, step, directory=""):
         _check(lib().scmf_write_density(self._h, str(directory).encode(), int(step)))
+
+    def write_density_async(self, step, directory=""):
+        """Queue density_<step>.dat from a device snapshot of the current
+        state; the file is written by a background thread."""
+        _check(lib().scmf_write_density_async(self._h, str(directory).encode(), int(step)))
+
+    def output_wait(self):
+        _check(lib().scmf_output_wait(self._h))
 
     def checkpoint(self, path, step):
         _check(lib().scmf_checkpoint_write(self._h, str(path).encode(), int(step)))

@@ -88,6 +88,31 @@ def test_density_writer_matches_reference_dump(golden, tmp_path):
         assert a.endswith(" ") and b.endswith(" ")
 
 
+def test_density_file_over_several_format_blocks(tmp_path):
+    """153,600 cells: the writer formats blocks of 65,536 rows on several
+    threads; rows must come out in cell order, each "%g " per value."""
+    nx, ny, nz, T = 64, 48, 50, 2
+    cells = nx * ny * nz
+    rng = np.random.default_rng(3)
+    counts = rng.integers(0, 300, size=T * cells).astype(np.uint32)
+    omega = (rng.standard_normal(T * cells) * 3).astype(np.float32)
+    n = (C.c_int32 * 3)(nx, ny, nz)
+    L = (C.c_double * 3)(6.4, 4.8, 5.0)
+    out = tmp_path / "density_7.dat"
+    fs = 0.0625
+    rc = scmf.lib().scmf_write_density_host(str(out).encode(), n, L, T, counts.ctypes.data,
+                                            omega.ctypes.data, C.c_double(fs))
+    assert rc == 0
+    got = out.read_text().split("\n")
+    assert got[0] == f"Nx={nx}, Ny={ny}, Nz={nz}"
+    assert len(got) == 2 + cells + 1 and got[-1] == ""
+    for c in (0, 1, 65535, 65536, 65537, 131071, 131072, cells - 1):
+        vals = [counts[c + t * cells] * fs for t in range(T)] + \
+               [float(omega[c + t * cells]) for t in range(T)]
+        assert got[2 + c] == "".join("%g " % v for v in vals), c
+    assert not os.path.exists(str(out) + ".tmp")
+
+
 def test_density_file_names():
     L = scmf.lib()
     # the name rule of io.cpp:16-19 is exercised through the host writer path
