@@ -26,6 +26,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def spec_pow2(spec):
+    """Every axis has cells of a power-of-two length (the engine's fast binning)."""
+    return all(math.frexp(L / n)[0] == 0.5 for L, n in zip(spec.L, spec.n))
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -214,6 +219,7 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
     g.reset_timing()
     g.reset_stats()
     g.sync()
+    cell_grids0 = g.stats().cell_grids
     t0 = time.perf_counter()
     for i in range(steps):
         g.step(warmup + i + 1)
@@ -221,6 +227,8 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
     t1 = time.perf_counter()
     tm = g.timing()
     st = g.stats()
+    # every timed gridding read the bead cells the sweep stored
+    from_cells = st.cell_grids - cell_grids0 == steps
     host = None
     if with_host_copy:
         # The rate a caller would see if it copied the positions out and back
@@ -285,8 +293,11 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
 
     B, TC = spec.n_beads, spec.n_types * spec.n_cells
     s = 2 if bits == 16 else 4
-    sweep_bytes = 24 * B + 4 * TC       # positions read + written, omega read once
-    grid_bytes = 12 * B + s * TC        # positions read, counts written
+    # positions read + written, omega read once (+ the bead cells written
+    # for the gridding); the gridding reads the cells (4 B) or the positions
+    # (12 B) and writes the counts
+    sweep_bytes = 24 * B + 4 * TC + (4 * B if from_cells else 0)
+    grid_bytes = (4 if from_cells else 12) * B + s * TC
     omega_bytes = s * TC + 4 * TC       # counts read, fp32 omega written
     tiled = spec.n_beads >= (1 << 23)
     sweep_kernel = {-1: "k_sweep_global", 1: "k_sweep_lds"}.get(lanes, f"k_sweep_sched<{lanes}>")
@@ -300,7 +311,11 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
             measured = json.load(f).get("bytes_per_launch", {})
     byte_t = "unsigned char" if spec.polymers and max(p.length for p in spec.polymers) <= 255 else "unsigned short"
     sweep_key = f"k_sweep_sched<{lanes}, {byte_t}>" if lanes > 1 else sweep_kernel
-    grid_keys = [f"k_grid_tiles<{bits}, true>", f"k_tiles_sum<{bits}>", "k_zero_counts"]
+    omega_key = (f"k_omega4<{spec.n_types}, {bits}>" if (spec.n_cells % 4 == 0)
+                 else "k_omega")
+    grid_keys = [f"k_grid_tiles<{bits}, false, true>" if from_cells
+                 else f"k_grid_tiles<{bits}, {'true' if spec_pow2(spec) else 'false'}, false>",
+                 f"k_tiles_sum<{bits}>", "k_zero_counts"]
 
     def traffic_of(keys):
         if all(k in measured for k in keys):
@@ -332,10 +347,11 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
                          kernel=sweep_kernel),
         "passes": {"grid": dict(roof(grid_bytes, tm.grid_ms,
                                      traffic_of(grid_keys) if tiled else None),
-                                kernel="k_zero_counts+k_grid_tiles+k_tiles_sum" if tiled
-                                else "k_zero_counts+k_grid"),
-                   "omega": dict(roof(omega_bytes, tm.omega_ms, traffic_of(["k_omega"])),
-                                 kernel="k_omega")},
+                                kernel=("k_zero_counts+k_grid_tiles(cells)+k_tiles_sum"
+                                        if from_cells else "k_zero_counts+k_grid_tiles+k_tiles_sum")
+                                if tiled else "k_zero_counts+k_grid"),
+                   "omega": dict(roof(omega_bytes, tm.omega_ms, traffic_of([omega_key])),
+                                 kernel=omega_key)},
         "acceptance": st.accepts / max(st.moves, 1),
         "grid_fallback_beads": int(st.grid_fallback),
         "pcie_inclusive": host,

@@ -89,6 +89,8 @@ typedef struct scmf_stats_t {
   int32_t overflow;      /* 1 if a uint16 count saturated since the last reset */
   uint64_t grid_fallback; /* beads the tiled gridding counted with global atomics */
   uint64_t sorts;         /* spatial reorderings done */
+  uint64_t cell_grids;    /* griddings that read the bead cells stored by the sweep
+                             instead of binning positions (tiled systems) */
 } scmf_stats_t;
 
 typedef struct scmf_timing_t {

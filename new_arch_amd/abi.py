@@ -64,6 +64,7 @@ class Stats(C.Structure):
         ("overflow", C.c_int32),
         ("grid_fallback", C.c_uint64),
         ("sorts", C.c_uint64),
+        ("cell_grids", C.c_uint64),
     ]
 
 

@@ -74,6 +74,8 @@ struct KParams {
   unsigned long long* accepts;    // ACC_SLOTS partial counts (summed on the host)
   uint32_t* overflow;
   uint32_t* ctr;                 // [0] stream sweep word of the next sweep
+  uint32_t* cells;               // packed cell of every stored bead, written by the sweep
+                                 // (x | y << 10 | z << 20), or null
 };
 
 // Accepted moves are counted in this many separate words, chosen by

@@ -14,6 +14,7 @@
 //   k_zero         clear the counts (a kernel, not a memset, so it can sit in a graph)
 //   k_grid         one lane per bead: NGP cell, atomic increment
 //   k_omega        one lane per cell: all T omega values in fp64, stored fp32
+//   k_omega4       the same, four cells per lane with vector loads and stores
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 #include <stdio.h>
@@ -474,10 +475,27 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
     }
     __syncthreads();
   }
-  for (int b = r; b < length; b += L) {
-    x[off + b] = sx[b * G + slot];
-    y[off + b] = sy[b * G + slot];
-    z[off + b] = sz[b * G + slot];
+  // write the chain back, with the packed cell of every bead for the gridding
+  SCMF_GLOBAL uint32_t* __restrict__ cells = gptr(kp.cells);
+  if (cells) {
+    const Axis ax0 = axis(kp, 0), ax1 = axis(kp, 1), ax2 = axis(kp, 2);
+    for (int b = r; b < length; b += L) {
+      const float px = sx[b * G + slot], py = sy[b * G + slot], pz = sz[b * G + slot];
+      x[off + b] = px;
+      y[off + b] = py;
+      z[off + b] = pz;
+      int xi, yi, zi;
+      const bool quick = kp.pow2 && (fold_pow2(px, ax0, &xi) & fold_pow2(py, ax1, &yi) &
+                                     fold_pow2(pz, ax2, &zi));
+      if (!quick) fold3_general(px, py, pz, ax0, ax1, ax2, &xi, &yi, &zi);
+      cells[off + b] = pack_cell(xi, yi, zi);
+    }
+  } else {
+    for (int b = r; b < length; b += L) {
+      x[off + b] = sx[b * G + slot];
+      y[off + b] = sy[b * G + slot];
+      z[off + b] = sz[b * G + slot];
+    }
   }
   wave_count(acc, (unsigned long long*)kp.accepts);
 }
@@ -567,6 +585,88 @@ __global__ void __launch_bounds__(BLOCK) k_omega(const KParams* __restrict__ Ps,
       }
       omega[c + (int64_t)t * C] = (float)w;
     }
+  }
+}
+
+// The same update for one system whose cell count is a multiple of four:
+// four cells per lane, the counts of each type from one 8- or 16-byte load
+// and the omega of each type stored with one 16-byte store.  Each cell's
+// arithmetic is k_omega's, in the same order (same bits).
+typedef uint32_t u32v2 __attribute__((ext_vector_type(2)));
+
+template <int T, class KP>
+__device__ __forceinline__ float omega_one(const KP& kp, const double (&n)[T], int t) {
+  double w = 0;
+#pragma unroll
+  for (int s = 0; s < T; ++s) w += n[s] * kp.fst[t];
+  w += -1;
+  w = kp.inv_nref * (kp.xn[t + T * t] * w);
+#pragma unroll
+  for (int s = 0; s < T; ++s) {
+    if (t != s) {
+      const double dnorm = -0.5 * kp.xn[t + T * s];
+      const double interaction = dnorm * kp.inv_nref * (kp.fst[t] * n[t] - kp.fst[s] * n[s]);
+      w += interaction;
+    }
+  }
+  return (float)w;
+}
+
+template <int T, int BITS>
+__global__ void __launch_bounds__(BLOCK) k_omega4(const KParams* __restrict__ Ps, int cadence) {
+  KPC& kp = param(Ps, 0);
+  if (cadence && !kp.omega_every_step && ((const uint32_t*)kp.ctr)[0] % 10u == 0u) return;
+  const SCMF_GLOBAL uint32_t* __restrict__ words = gptr((const uint32_t*)kp.counts);
+  SCMF_GLOBAL float* __restrict__ omega = gptr((float*)kp.omega);
+  const uint32_t C = kp.n_cells;
+  const uint32_t groups = C / 4;
+  const uint32_t stride = gridDim.x * BLOCK;
+  for (uint32_t gi = blockIdx.x * BLOCK + threadIdx.x; gi < groups; gi += stride) {
+    uint32_t raw[T][BITS == 32 ? 4 : 2];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      const uint32_t first = (uint32_t)t * C + 4 * gi;  // a multiple of four
+      if constexpr (BITS == 32) {
+        const u32v4 v = *(const SCMF_GLOBAL u32v4*)(words + first);
+        raw[t][0] = v.x;
+        raw[t][1] = v.y;
+        raw[t][2] = v.z;
+        raw[t][3] = v.w;
+      } else {
+        const u32v2 v = *(const SCMF_GLOBAL u32v2*)(words + first / 2);
+        raw[t][0] = v.x;
+        raw[t][1] = v.y;
+      }
+    }
+    float out[T][4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      double n[T];
+#pragma unroll
+      for (int t = 0; t < T; ++t)
+        n[t] = (double)(BITS == 32 ? raw[t][e] : (raw[t][e >> 1] >> ((e & 1) * 16)) & 0xFFFFu);
+#pragma unroll
+      for (int t = 0; t < T; ++t) out[t][e] = omega_one<T>(kp, n, t);
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+      f32x4 v = {out[t][0], out[t][1], out[t][2], out[t][3]};
+      *(SCMF_GLOBAL f32x4*)(omega + (uint32_t)t * C + 4 * gi) = v;
+    }
+  }
+}
+
+template <int BITS>
+const void* omega4_kernel(int T) {
+  switch (T) {
+    case 1: return (const void*)k_omega4<1, BITS>;
+    case 2: return (const void*)k_omega4<2, BITS>;
+    case 3: return (const void*)k_omega4<3, BITS>;
+    case 4: return (const void*)k_omega4<4, BITS>;
+    case 5: return (const void*)k_omega4<5, BITS>;
+    case 6: return (const void*)k_omega4<6, BITS>;
+    case 7: return (const void*)k_omega4<7, BITS>;
+    default: return (const void*)k_omega4<8, BITS>;
   }
 }
 
@@ -836,12 +936,15 @@ struct scmf_sys {
   float* y2 = nullptr;
   float* z2 = nullptr;
   uint16_t* scratch = nullptr;  // window images of the tiled gridding
+  uint32_t* cells = nullptr;    // packed bead cells written by the sweep (tiled systems)
+  bool cells_fresh = false;     // cells match the positions (the last change was such a sweep)
   unsigned long long* d_fallback = nullptr;
   TileGeo geo{};
   size_t tile_lds = 0;  // LDS of k_grid_tiles; 0 = tiled gridding not usable
   bool sorted = false;
   int sort_every = 0;
   uint64_t n_sorts = 0;
+  uint64_t n_cell_grids = 0;
   KParams* d_kp = nullptr;  // device copy of kp read by the batch-capable kernels
   uint32_t* d_ctr = nullptr;
   size_t lds_bytes = 0; // LDS of the sweep kernel
@@ -1254,21 +1357,30 @@ int alloc_sort_buffers(scmf_sys* s) {
       (rc = A((void**)&s->scratch,
               sizeof(uint16_t) * ((size_t)s->geo.ntiles * s->geo.slot_elems + 64))))
     return rc;
+  // the sweep stores bead cells for the next gridding where 10 bits per axis hold them
+  if (s->kp.nx <= CELL_AXIS_MAX && s->kp.ny <= CELL_AXIS_MAX && s->kp.nz <= CELL_AXIS_MAX) {
+    if ((rc = A((void**)&s->cells, sizeof(uint32_t) * padded(B)))) return rc;
+    HIP_TRY(hipMemsetAsync(s->cells, 0, sizeof(uint32_t) * padded(B), s->stream));
+    s->kp.cells = s->cells;
+    s->cells_fresh = false;
+  }
   // the window sums read absent contributions from the element after the
   // last image, which nothing writes
   HIP_TRY(hipMemsetAsync(s->scratch + (size_t)s->geo.ntiles * s->geo.slot_elems, 0,
                          sizeof(uint16_t) * 64, s->stream));
-  if (s->tile_lds + MONO_LDS + 128 > 65536 &&
-      (hipFuncSetAttribute((const void*)k_grid_tiles<16, false>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) != hipSuccess ||
-       hipFuncSetAttribute((const void*)k_grid_tiles<32, false>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) != hipSuccess ||
-       hipFuncSetAttribute((const void*)k_grid_tiles<16, true>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) != hipSuccess ||
-       hipFuncSetAttribute((const void*)k_grid_tiles<32, true>,
-                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) != hipSuccess)) {
-    (void)hipGetLastError();
-    return fail(SCMF_E_HIP, "cannot raise the LDS limit of the tiled gridding");
+  if (s->tile_lds + MONO_LDS + 128 > 65536) {
+    const void* fns[] = {(const void*)k_grid_tiles<16, false, false>,
+                         (const void*)k_grid_tiles<32, false, false>,
+                         (const void*)k_grid_tiles<16, true, false>,
+                         (const void*)k_grid_tiles<32, true, false>,
+                         (const void*)k_grid_tiles<16, false, true>,
+                         (const void*)k_grid_tiles<32, false, true>};
+    for (const void* f : fns)
+      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) !=
+          hipSuccess) {
+        (void)hipGetLastError();
+        return fail(SCMF_E_HIP, "cannot raise the LDS limit of the tiled gridding");
+      }
   }
   return SCMF_OK;
 }
@@ -1300,6 +1412,7 @@ int sort_chains(scmf_sys* s) {
   s->kp.x = s->x;
   s->kp.y = s->y;
   s->kp.z = s->z;
+  s->cells_fresh = false;  // positions permuted, cells not
   if ((rc = sync_kp(s))) return rc;
   s->sorted = true;
   s->n_sorts += 1;
@@ -1308,11 +1421,15 @@ int sort_chains(scmf_sys* s) {
 
 // The tiled gridding of one system whose parameters sit at Ps[sys] (the
 // counts must already be cleared).
-int launch_tiled_grid(scmf_sys* s, hipStream_t st, const KParams* Ps, int sys) {
+// use_cells: read the bead cells the last sweep stored (valid only while
+// s->cells_fresh holds) instead of binning the positions.
+int launch_tiled_grid(scmf_sys* s, hipStream_t st, const KParams* Ps, int sys, bool use_cells) {
   const dim3 tiles((unsigned)s->geo.ntiles);
   const bool p2 = s->kp.pow2 != 0;
-  auto bin = s->opts.density_bits == 32 ? (p2 ? k_grid_tiles<32, true> : k_grid_tiles<32, false>)
-                                         : (p2 ? k_grid_tiles<16, true> : k_grid_tiles<16, false>);
+  const bool b32 = s->opts.density_bits == 32;
+  auto bin = use_cells ? (b32 ? k_grid_tiles<32, false, true> : k_grid_tiles<16, false, true>)
+             : b32     ? (p2 ? k_grid_tiles<32, true, false> : k_grid_tiles<32, false, false>)
+                       : (p2 ? k_grid_tiles<16, true, false> : k_grid_tiles<16, false, false>);
   auto sum = s->opts.density_bits == 32 ? k_tiles_sum<32> : k_tiles_sum<16>;
   hipLaunchKernelGGL(bin, tiles, dim3(GRID_THREADS), s->tile_lds, st, Ps, sys, s->geo,
                      (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
@@ -1331,7 +1448,7 @@ int free_all(scmf_sys* s) {
                   (void*)s->tstart, (void*)s->tfill, (void*)s->new_slot, (void*)s->x2,
                   (void*)s->y2, (void*)s->z2, (void*)s->scratch, (void*)s->d_fallback,
                   (void*)s->d_kp, (void*)s->d_ctr, (void*)s->fft_buf, (void*)s->d_sq_sum,
-                  (void*)s->d_sq_cnt})
+                  (void*)s->d_sq_cnt, (void*)s->cells})
     if (p) (void)hipFree(p);
   if (s->fft_ready) (void)hipfftDestroy(s->fft);
   for (Timed& t : s->pending) {
@@ -1719,6 +1836,7 @@ int scmf_mc_sweep(scmf_sys* s, int32_t n_sweeps) {
     int rc = launch_sweep(s);
     if (rc) return rc;
     HIP_TRY(hipGetLastError());
+    s->cells_fresh = s->kp.cells != nullptr && s->variant->L >= 2;
     if (s->timing) {
       int rc = s->ev_end(&t);
       if (rc) return rc;
@@ -1744,7 +1862,8 @@ int scmf_grid_density(scmf_sys* s) {
                      (const KParams*)s->d_kp, (const BlockRef*)nullptr);
   const bool b32 = s->opts.density_bits == 32;
   if (s->tile_lds && s->sorted) {
-    int rc = launch_tiled_grid(s, s->stream, s->d_kp, 0);
+    if (s->cells_fresh) s->n_cell_grids += 1;
+    int rc = launch_tiled_grid(s, s->stream, s->d_kp, 0, s->cells_fresh);
     if (rc) return rc;
   } else {
     hipLaunchKernelGGL(k_grid, dim3(grid_strided(s->n_beads)), dim3(BLOCK), 0, s->stream,
@@ -1767,8 +1886,17 @@ int scmf_update_omega(scmf_sys* s) {
     int rc = s->ev_begin(K_OMEGA, &t);
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_omega, dim3(grid_strided(s->n_cells)), dim3(BLOCK), 0, s->stream,
-                     (const KParams*)s->d_kp, (const BlockRef*)nullptr, 0);
+  if (s->n_cells % 4 == 0) {
+    const void* fn = s->opts.density_bits == 32 ? omega4_kernel<32>(s->T) : omega4_kernel<16>(s->T);
+    const KParams* kpp = s->d_kp;
+    int cadence = 0;
+    void* args[] = {(void*)&kpp, (void*)&cadence};
+    HIP_TRY(hipLaunchKernel(fn, dim3(grid_strided(s->n_cells / 4)), dim3(BLOCK), args, 0,
+                            s->stream));
+  } else {
+    hipLaunchKernelGGL(k_omega, dim3(grid_strided(s->n_cells)), dim3(BLOCK), 0, s->stream,
+                       (const KParams*)s->d_kp, (const BlockRef*)nullptr, 0);
+  }
   HIP_TRY(hipGetLastError());
   if (s->timing) return s->ev_end(&t);
   return SCMF_OK;
@@ -1834,6 +1962,7 @@ int scmf_copy_out(scmf_sys* s, int32_t what, void* host, size_t bytes) {
 
 int scmf_copy_in_pos(scmf_sys* s, const float* xyz, size_t n_beads) {
   if (!s || !xyz) return fail(SCMF_E_ARG, "null argument");
+  s->cells_fresh = false;
   if ((int64_t)n_beads != s->n_beads) return fail(SCMF_E_ARG, "bead count mismatch");
   int rc = ensure_staging(s, std::min<int64_t>(s->n_beads, STAGE_BEADS));
   if (rc) return rc;
@@ -1985,6 +2114,7 @@ int scmf_stats(scmf_sys* s, scmf_stats_t* out) {
   out->overflow = ovf ? 1 : 0;
   out->grid_fallback = fb;
   out->sorts = s->n_sorts;
+  out->cell_grids = s->n_cell_grids;
   return SCMF_OK;
 }
 
@@ -2081,6 +2211,7 @@ int scmf_autotune(scmf_sys* s, int32_t sweeps_per_variant, int32_t* chosen, floa
                          hipMemcpyHostToDevice, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   (void)hipFree(backup);
+  s->cells_fresh = false;  // positions restored
   s->sweeps = sweeps0;
   s->moves = moves0;
   s->opts.lanes_per_chain = keep_req;
@@ -2397,7 +2528,10 @@ int batch_build(scmf_batch* b) {
   batch_free_tables(b);
   const int n = (int)b->m.size();
   std::vector<KParams> ps(n);
-  for (int i = 0; i < n; ++i) ps[i] = b->m[i]->kp;
+  for (int i = 0; i < n; ++i) {
+    ps[i] = b->m[i]->kp;
+    ps[i].cells = nullptr;  // the batch grids from positions
+  }
   HIP_TRY(hipMemcpy(b->d_Ps, ps.data(), sizeof(KParams) * n, hipMemcpyHostToDevice));
   // sweep groups by kernel shape; members ordered by decreasing work first
   std::vector<int> order(n);
@@ -2485,7 +2619,8 @@ int batch_build(scmf_batch* b) {
     hipLaunchKernelGGL(k_grid, dim3(b->n_grid), dim3(BLOCK), 0, b->stream, Ps,
                        (const BlockRef*)b->d_bt_grid);
   for (int i = 0; i < n; ++i)
-    if (b->m[i]->tile_lds && b->m[i]->sorted) (void)launch_tiled_grid(b->m[i], b->stream, Ps, i);
+    if (b->m[i]->tile_lds && b->m[i]->sorted)
+      (void)launch_tiled_grid(b->m[i], b->stream, Ps, i, false);
   hipLaunchKernelGGL(k_omega, dim3(b->n_omega), dim3(BLOCK), 0, b->stream, Ps,
                      (const BlockRef*)b->d_bt_omega, 1);
   hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
@@ -2562,13 +2697,17 @@ int scmf_batch_step(scmf_batch* b, int32_t n_steps) {
     }
     if (rebuilt) {
       std::vector<KParams> ps(b->m.size());
-      for (size_t i = 0; i < b->m.size(); ++i) ps[i] = b->m[i]->kp;
+      for (size_t i = 0; i < b->m.size(); ++i) {
+        ps[i] = b->m[i]->kp;
+        ps[i].cells = nullptr;
+      }
       HIP_TRY(hipMemcpyAsync(b->d_Ps, ps.data(), sizeof(KParams) * ps.size(),
                              hipMemcpyHostToDevice, b->stream));
       HIP_TRY(hipStreamSynchronize(b->stream));
     }
     HIP_TRY(hipGraphLaunch(b->exec, b->stream));
     for (scmf_sys* s : b->m) {
+      s->cells_fresh = false;
       s->sweeps += 1;
       s->moves += (uint64_t)s->n_beads;
     }

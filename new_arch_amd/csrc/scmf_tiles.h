@@ -175,6 +175,13 @@ __device__ __forceinline__ int wrap_rel(int c, int origin, int n) {
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32v4 __attribute__((ext_vector_type(4)));
+
+// Packed cell of a bead, as the sweep stores it: 10 bits per axis.
+constexpr int CELL_AXIS_MAX = 1024;
+__device__ __forceinline__ uint32_t pack_cell(int xi, int yi, int zi) {
+  return (uint32_t)xi | ((uint32_t)yi << 10) | ((uint32_t)zi << 20);
+}
 
 constexpr int MONO_LDS = 4096;     // architectures up to this length read types from LDS
 constexpr uint32_t OUT_OF_WINDOW = 1u << 30;  // the sum of three table entries stays below 2^32
@@ -247,7 +254,9 @@ __device__ void normalise_window(uint32_t* win, const TileGeo& g, KPC& kp, int o
 // straight to the final counts with a global atomic.  Positions are read
 // four beads per lane (16-byte loads; the arrays are padded to a multiple of
 // four beads), the next four in flight while the current ones are binned.
-template <int BITS, bool POW2>
+// With CELLS the cells the last sweep stored for the same positions are read
+// instead (4 bytes a bead, no binning).
+template <int BITS, bool POW2, bool CELLS>
 __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __restrict__ Ps, int sys,
                                                              TileGeo g,
                                                              const uint32_t* __restrict__ tile_start,
@@ -266,6 +275,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   const SCMF_GLOBAL float* __restrict__ x = gptr((const float*)kp.x);
   const SCMF_GLOBAL float* __restrict__ y = gptr((const float*)kp.y);
   const SCMF_GLOBAL float* __restrict__ z = gptr((const float*)kp.z);
+  const SCMF_GLOBAL uint32_t* __restrict__ cellw = gptr((const uint32_t*)kp.cells);
   const SCMF_GLOBAL uint8_t* __restrict__ btype = gptr((const uint8_t*)kp.btype);
   SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
   uint32_t* __restrict__ overflow = (uint32_t*)kp.overflow;
@@ -336,19 +346,32 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
         if (m0 >= len) m0 -= len;
       }
       const int64_t a0 = gq < b1 ? gq : last;
-      f32x4 X = *(const SCMF_GLOBAL f32x4*)(x + a0);
-      f32x4 Y = *(const SCMF_GLOBAL f32x4*)(y + a0);
-      f32x4 Z = *(const SCMF_GLOBAL f32x4*)(z + a0);
+      f32x4 X = {}, Y = {}, Z = {};
+      u32v4 Q = {};
+      if constexpr (CELLS) {
+        Q = *(const SCMF_GLOBAL u32v4*)(cellw + a0);
+      } else {
+        X = *(const SCMF_GLOBAL f32x4*)(x + a0);
+        Y = *(const SCMF_GLOBAL f32x4*)(y + a0);
+        Z = *(const SCMF_GLOBAL f32x4*)(z + a0);
+      }
       for (int64_t base = b0 & ~(int64_t)3; base < b1; base += PASS_BEADS) {
         const int64_t gn = gq + PASS_BEADS;
         const int64_t an = gn < b1 ? gn : last;
-        const f32x4 Xn = *(const SCMF_GLOBAL f32x4*)(x + an);
-        const f32x4 Yn = *(const SCMF_GLOBAL f32x4*)(y + an);
-        const f32x4 Zn = *(const SCMF_GLOBAL f32x4*)(z + an);
+        f32x4 Xn = {}, Yn = {}, Zn = {};
+        u32v4 Qn = {};
+        if constexpr (CELLS) {
+          Qn = *(const SCMF_GLOBAL u32v4*)(cellw + an);
+        } else {
+          Xn = *(const SCMF_GLOBAL f32x4*)(x + an);
+          Yn = *(const SCMF_GLOBAL f32x4*)(y + an);
+          Zn = *(const SCMF_GLOBAL f32x4*)(z + an);
+        }
         if (gq < b1) {
           int m = m0;
           const float px[4] = {X.x, X.y, X.z, X.w}, py[4] = {Y.x, Y.y, Y.z, Y.w},
                       pz[4] = {Z.x, Z.y, Z.z, Z.w};
+          const uint32_t pq[4] = {Q.x, Q.y, Q.z, Q.w};
           // beads e in [elo, ehi) of the four belong to the range
           const int elo = gq < b0 ? (int)(b0 - gq) : 0;
           const int ehi = b1 - gq < 4 ? (int)(b1 - gq) : 4;
@@ -357,9 +380,15 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
             if (e >= elo && e < ehi) {
               const uint32_t ty_ = TAB ? (uint32_t)mono_s[m] : (uint32_t)btype[gq + e];
               int xi, yi, zi;
-              const bool quick = POW2 && (fold_pow2(px[e], ax0, &xi) & fold_pow2(py[e], ax1, &yi) &
-                                          fold_pow2(pz[e], ax2, &zi));
-              if (!quick) fold3_general(px[e], py[e], pz[e], ax0, ax1, ax2, &xi, &yi, &zi);
+              if constexpr (CELLS) {
+                xi = (int)(pq[e] & 1023u);
+                yi = (int)((pq[e] >> 10) & 1023u);
+                zi = (int)(pq[e] >> 20);
+              } else {
+                const bool quick = POW2 && (fold_pow2(px[e], ax0, &xi) &
+                                            fold_pow2(py[e], ax1, &yi) & fold_pow2(pz[e], ax2, &zi));
+                if (!quick) fold3_general(px[e], py[e], pz[e], ax0, ax1, ax2, &xi, &yi, &zi);
+              }
               const uint32_t sp = tab_x[xi] + tab_y[yi] + tab_z[zi];
               if (sp < OUT_OF_WINDOW) {
                 const uint32_t wc = __umul24(ty_, (uint32_t)g.win_cells) + sp;
@@ -373,9 +402,13 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
             if (++m == len) m = 0;
           }
         }
-        X = Xn;
-        Y = Yn;
-        Z = Zn;
+        if constexpr (CELLS) {
+          Q = Qn;
+        } else {
+          X = Xn;
+          Y = Yn;
+          Z = Zn;
+        }
         gq = gn;
         m0 += step_m;
         if (m0 >= len) m0 -= len;

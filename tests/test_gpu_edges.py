@@ -138,3 +138,51 @@ def test_crowded_cell(bits):
         assert g.stats().sorts >= 1
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("L,lanes", [(4.0, 0), (3.3, 8)])
+def test_gridding_follows_every_position_change(tmp_path, L, lanes):
+    """A tiled system grids from the bead cells its last sweep stored; every
+    other change of positions (set_positions, ordering, checkpoint restore,
+    autotune) makes it bin positions again.  Counts equal the oracle's after
+    each, and cell_grids counts the griddings that read cells.  L = 4 gives
+    power-of-two cells, L = 3.3 the general binning in the sweep."""
+    banner(f"cells follow positions, L={L}")
+    spec = W.tiny(n_chains=3000, N=32, grid=(32, 32, 32), L=(L, L, L))
+    g = scmf.System(spec, make_opts(seed=17, density_bits=32, sort_every=1,
+                                    lanes_per_chain=lanes))
+    try:
+        o = orc.Oracle(spec, g.opts)
+
+        def check(cell_grids):
+            g.grid_density()
+            assert np.array_equal(g.density(), o.grid(g.positions()))
+            assert g.stats().cell_grids == cell_grids
+
+        check(0)                                   # initial coordinates
+        g.update_omega()
+        g.step(1)                                  # order, sweep, grid from cells
+        assert np.array_equal(g.density(), o.grid(g.positions()))
+        assert g.stats().cell_grids == 1
+        g.mc_sweep(1)
+        check(2)
+        pos = g.positions()
+        pos[::7] += np.float32(0.37)               # some beads leave their cells
+        g.set_positions(pos)
+        check(2)
+        g.mc_sweep(1)
+        g.sort_chains()                            # slots permuted, cells not
+        check(2)
+        g.mc_sweep(1)
+        ck = tmp_path / "c.ckpt"
+        g.checkpoint(ck, 3)
+        g.mc_sweep(2)
+        assert g.restore(ck) == 3
+        check(2)
+        g.mc_sweep(1)
+        tuned, _ = g.autotune(1)                   # tries every shape, restores positions
+        check(2)
+        g.mc_sweep(1)
+        check(3 if tuned >= 2 else 2)              # only the scheduled shapes store cells
+    finally:
+        g.close()
