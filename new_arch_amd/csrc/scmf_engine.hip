@@ -1226,13 +1226,6 @@ int launch_sweep(scmf_sys* s) {
   return SCMF_OK;
 }
 
-// Tile geometry of the tiled gridding.  The halo covers the reach of a chain
-// from its middle bead (0.75 Re sqrt((N-1)/(Nref-1)), in cells).  Per axis the
-// tile edge ts is the largest that keeps the window (ts + 2h per axis) inside
-// the LDS budget and the grid, with every tile (the last, partial one
-// included) at least h cells long, so a cell is covered only by the windows of
-// its own tile and the two next to it.  The budget is the one maximising
-// interior/window volume times resident workgroups per CU.
 // Tile geometry of the tiled gridding.  The halo covers the reach of a
 // typical chain from its middle bead (0.75 Re scaled by sqrt((N-1)/(Nref-1))
 // for the bead-weighted median length N); beads beyond it are counted with

@@ -324,9 +324,6 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
     // The type source is fixed per architecture; two copies of the loop keep
     // a global type load from sharing registers with the LDS one (which
     // would make every LDS read wait for the prefetched positions).
-    // The type source is fixed per architecture; two copies of the loop keep
-    // a global type load from sharing registers with the LDS one (which
-    // would make every LDS read wait for the prefetched positions).
     auto bin_range = [&](auto tab_c) {
       constexpr bool TAB = decltype(tab_c)::value;
       // Each lane takes four beads per pass and loads the next pass's four
