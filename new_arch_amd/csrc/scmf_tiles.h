@@ -303,9 +303,29 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   int64_t pop = 0;
   for (int p = 0; p < kp.P; ++p)
     pop += (int64_t)(tile_start[p * g.ntiles + t + 1] - tile_start[p * g.ntiles + t]) * kp.len[p];
+  // A tile of at most 65,535 beads cannot wrap a 16-bit window counter.  A
+  // larger tile is binned optimistically first: the atomics return the old
+  // words and every lane ORs them together.  A counter can only wrap after
+  // passing through 0x8000-0xFFFF, and some increment then sees bit 15 set
+  // (a carry out of a low half needs that half at 0xFFFF first), so if no
+  // lane saw bit 15 in either half the window is exact.  Otherwise the window
+  // is cleared and the tile binned again, normalising every NORM_BEADS beads
+  // so no half can wrap, with the beads outside the window (already counted
+  // globally) skipped.
   const bool checked = pop > 65535;
+  __shared__ uint32_t redo_s;
+  if (threadIdx.x == 0) redo_s = 0u;
   const Axis ax0 = axis(kp, 0), ax1 = axis(kp, 1), ax2 = axis(kp, 2);
   uint32_t missed = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+  const bool safe = attempt == 1;        // the normalising pass
+  const bool track = checked && !safe;   // returned words watched for bit 15
+  if (safe) {
+    __syncthreads();
+    if (redo_s == 0u) break;
+    for (int w = threadIdx.x; w < g.win_words_padded; w += GRID_THREADS) win[w] = 0u;
+  }
+  uint32_t seen = 0;
   int since = 0;
   for (int p = 0; p < kp.P; ++p) {
     const int64_t s0 = tile_start[p * g.ntiles + t], s1 = tile_start[p * g.ntiles + t + 1];
@@ -389,8 +409,11 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
               const uint32_t sp = tab_x[xi] + tab_y[yi] + tab_z[zi];
               if (sp < OUT_OF_WINDOW) {
                 const uint32_t wc = __umul24(ty_, (uint32_t)g.win_cells) + sp;
-                atomicAdd(&win[wc >> 1], 1u << ((wc & 1u) << 4));
-              } else {
+                if (track)
+                  seen |= atomicAdd(&win[wc >> 1], 1u << ((wc & 1u) << 4));
+                else
+                  atomicAdd(&win[wc >> 1], 1u << ((wc & 1u) << 4));
+              } else if (!safe) {
                 ++missed;
                 add_global<BITS>(counts, overflow,
                                  ty_ * kp.n_cells + (uint32_t)(xi + kp.nx * (yi + kp.ny * zi)), 1u);
@@ -409,7 +432,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
         gq = gn;
         m0 += step_m;
         if (m0 >= len) m0 -= len;
-        if (checked && (since += PASS_BEADS) >= NORM_BEADS) {
+        if (safe && (since += PASS_BEADS) >= NORM_BEADS) {
           since = 0;
           __syncthreads();
           normalise_window<BITS>(win, g, kp, ox, oy, oz, counts, overflow);
@@ -421,6 +444,11 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
       bin_range(std::true_type{});
     else
       bin_range(std::false_type{});
+  }
+  if (track) {
+    for (int off = 32; off > 0; off >>= 1) seen |= __shfl_xor(seen, off);
+    if ((threadIdx.x & 63) == 0 && (seen & 0x80008000u)) atomicOr(&redo_s, 1u);
+  }
   }
   // beads counted globally: one atomic per wave
   for (int off = 32; off > 0; off >>= 1) missed += __shfl_xor(missed, off);
