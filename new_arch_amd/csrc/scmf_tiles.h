@@ -468,20 +468,38 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
     const int n_r = kp.T * nrow;
     if (n_r == 0) continue;
     const float iex = 1.0f / (float)ex, iey = 1.0f / (float)ey, irow = 1.0f / (float)nrow;
-    uint16_t* __restrict__ dst = img + reg_s[r];
+    // Two elements per lane, stored as one word: a region starts at an even
+    // element of a 4-byte aligned image and is allocated with an even count
+    // (an odd region's last word carries a spare element nothing reads).
+    uint32_t* __restrict__ dst = (uint32_t*)(img + reg_s[r]);
     // (every product below is of values under 2^24)
     const uint32_t W0 = (uint32_t)g.W[0], W01 = (uint32_t)(g.W[0] * g.W[1]), wcl = (uint32_t)g.win_cells;
     const uint32_t base_off = (uint32_t)fx + W0 * (uint32_t)fy + W01 * (uint32_t)fz;
-    for (int e = threadIdx.x; e < n_r; e += GRID_THREADS) {
+    // window-cell steps from the last cell of a row to the first of the next
+    // row, plane and type
+    const uint32_t row_step = W0 - (uint32_t)(ex - 1);
+    const uint32_t plane_step = W01 - (uint32_t)(ex - 1) - W0 * (uint32_t)(ey - 1);
+    const uint32_t type_step = wcl - (uint32_t)(ex - 1) - W0 * (uint32_t)(ey - 1) -
+                               W01 * (uint32_t)(ez - 1);
+    const int n_pairs = (n_r + 1) >> 1;
+    for (int j = threadIdx.x; j < n_pairs; j += GRID_THREADS) {
+      const int e = 2 * j;
       const int type = small_div(e, irow);
       const int l = e - (int)__umul24((uint32_t)type, (uint32_t)nrow);
       const int lyz = small_div(l, iex);
       const int lx = l - (int)__umul24((uint32_t)lyz, (uint32_t)ex);
       const int lz = small_div(lyz, iey);
       const int ly = lyz - (int)__umul24((uint32_t)lz, (uint32_t)ey);
-      const uint32_t wc = __umul24((uint32_t)type, wcl) + base_off + (uint32_t)lx +
-                          __umul24(W0, (uint32_t)ly) + __umul24(W01, (uint32_t)lz);
-      dst[e] = (uint16_t)(win[wc >> 1] >> ((wc & 1u) << 4));
+      const uint32_t wc0 = __umul24((uint32_t)type, wcl) + base_off + (uint32_t)lx +
+                           __umul24(W0, (uint32_t)ly) + __umul24(W01, (uint32_t)lz);
+      const uint32_t wc1 = wc0 + (lx + 1 < ex   ? 1u
+                                  : ly + 1 < ey ? row_step
+                                  : lz + 1 < ez ? plane_step
+                                  : e + 1 < n_r ? type_step
+                                                : 0u);
+      const uint32_t v0 = (win[wc0 >> 1] >> ((wc0 & 1u) << 4)) & 0xFFFFu;
+      const uint32_t v1 = (win[wc1 >> 1] >> ((wc1 & 1u) << 4)) & 0xFFFFu;
+      dst[j] = v0 | (v1 << 16);
     }
   }
 }
