@@ -36,7 +36,8 @@ def log(msg):
 
 
 def cpu_baseline(spec, opts, target_s=15.0):
-    """Oracle sweep on the first chains of the workload, all host threads."""
+    """The oracle's step on all host threads: its sweep timed on the first
+    chains of the workload, its gridding and omega update on the whole system."""
     import numpy as np
 
     from new_arch_amd.abi import SystemSpec
@@ -67,9 +68,26 @@ def cpu_baseline(spec, opts, target_s=15.0):
         o.sweep(pos, om, s)
     dt = time.perf_counter() - t
     moves = chains * N * sweeps
-    return {"value": moves / dt, "unit": "bead-moves/s", "cores": orc.threads(), "kind": "port",
-            "sample": f"oracle smart-MC sweep of the first {chains} chains ({chains * N} beads) "
-                      f"x {sweeps} sweeps, {dt:.1f} s; gridding and omega not included"}
+    sweep_rate = moves / dt
+    # The gridding and the omega update of the whole system, once each.
+    o = orc.Oracle(spec, opts)
+    pos = o.init_coords()
+    t = time.perf_counter()
+    counts = o.grid(pos)
+    t_grid = time.perf_counter() - t
+    del pos
+    t = time.perf_counter()
+    o.omega(counts)
+    t_omega = time.perf_counter() - t
+    B = spec.n_beads
+    step_s = B / sweep_rate + t_grid + t_omega
+    return {"value": B / step_s, "unit": "bead-moves/s", "cores": orc.threads(), "kind": "port",
+            "sweep_bead_moves_per_s": sweep_rate, "grid_s": round(t_grid, 3),
+            "omega_s": round(t_omega, 4),
+            "sample": f"one step = sweep + gridding + omega: the oracle's smart-MC sweep of the "
+                      f"first {chains} chains ({chains * N} beads) x {sweeps} sweeps ({dt:.1f} s) "
+                      f"scaled to {B} beads, plus its gridding ({t_grid:.2f} s) and omega update "
+                      f"({t_omega:.3f} s) of the whole system"}
 
 
 def batched_vs_plain(steps, warmup, n_systems=64):
