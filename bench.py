@@ -238,6 +238,7 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
     g.reset_stats()
     g.sync()
     cell_grids0 = g.stats().cell_grids
+    sorts0 = g.stats().sorts
     t0 = time.perf_counter()
     for i in range(steps):
         g.step(warmup + i + 1)
@@ -372,6 +373,8 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
                                  kernel=omega_key)},
         "acceptance": st.accepts / max(st.moves, 1),
         "grid_fallback_beads": int(st.grid_fallback),
+        # spatial orderings inside the timed steps (one every 25 sweeps)
+        "orderings_timed": int(st.sorts - sorts0),
         "pcie_inclusive": host,
         "analysis": analysis,
     }
@@ -380,7 +383,7 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=25)  # one spatial ordering period
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="large_diblock_melt")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -414,6 +417,7 @@ def main():
         "roofline": m["roofline"],
         "passes": m["passes"],
         "acceptance": m["acceptance"],
+        "orderings_timed": m["orderings_timed"],
         "pcie_inclusive": m["pcie_inclusive"],
         "analysis": m["analysis"],
     }
