@@ -63,6 +63,28 @@ __device__ __forceinline__ int small_div(int a, float inv_b) {
 
 // ---------------------------------------------------------------- ordering
 // Key of every slot: architecture block, then tile of the chain's middle bead.
+// Adds one to counter[key] for every active lane of the wave with one atomic
+// per distinct key; returns the counter's value before the lane's own unit
+// (the old value plus the lane's rank among the lanes with the same key).
+// Slots in the previous order are mostly grouped by tile already, so a wave
+// usually holds one or two keys and the tile counters see few atomics.
+__device__ __forceinline__ uint32_t wave_add_one(uint32_t* counter, uint32_t key) {
+  const int lane = (int)(threadIdx.x & 63u);
+  unsigned long long todo = __ballot(1);
+  uint32_t mine = 0u;
+  while (todo) {
+    const int leader = __ffsll(todo) - 1;
+    const uint32_t k = __shfl(key, leader);
+    const unsigned long long same = __ballot(key == k);
+    uint32_t base = 0u;
+    if (lane == leader) base = atomicAdd(&counter[k], (uint32_t)__popcll(same));
+    base = __shfl(base, leader);
+    if (key == k) mine = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+    todo &= ~same;
+  }
+  return mine;
+}
+
 __global__ void __launch_bounds__(256) k_tile_keys(KParams kp, TileGeo g,
                                                    const float* __restrict__ x,
                                                    const float* __restrict__ y,
@@ -81,7 +103,7 @@ __global__ void __launch_bounds__(256) k_tile_keys(KParams kp, TileGeo g,
                                            g.nt[1] * tile_index(g.q[2], g.rem[2], zi)));
   const uint32_t key = (uint32_t)p * (uint32_t)g.ntiles + t;
   keys[s] = key;
-  atomicAdd(&count[key], 1u);
+  (void)wave_add_one(count, key);
 }
 
 // Exclusive scan of n counts into start[0..n] (one workgroup of 1024 lanes),
@@ -121,7 +143,7 @@ __global__ void __launch_bounds__(256) k_tile_scatter(int64_t n, const uint32_t*
   const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (s >= n) return;
   const uint32_t k = keys[s];
-  new_slot[s] = start[k] + atomicAdd(&fill[k], 1u);
+  new_slot[s] = start[k] + wave_add_one(fill, k);
 }
 
 // Move every chain to its new slot (positions double-buffered) and update
