@@ -2025,7 +2025,7 @@ int scmf_structure_factor(scmf_sys* s, const double* coef, double dq, int32_t by
   const int64_t hx = nx / 2 + 1;
   if (!s->fft_ready) {
     const size_t bytes = (size_t)nz * ny * hx * 2 * sizeof(double);
-    HIP_TRY(hipMalloc(&s->fft_buf, bytes));
+    if (!s->fft_buf) HIP_TRY(hipMalloc(&s->fft_buf, bytes));
     hipfftHandle h = 0;
     if (hipfftCreate(&h) != HIPFFT_SUCCESS) return fail(SCMF_E_HIP, "hipfftCreate failed");
     long long n[3] = {nz, ny, nx};
