@@ -194,6 +194,10 @@ int scmf_write_density(scmf_sys* sys, const char* dir, int32_t step);
    write error is reported by the next call or by scmf_output_wait.  No
    counterpart in the reference, whose dumps are synchronous. */
 int scmf_write_density_async(scmf_sys* sys, const char* dir, int32_t step);
+/* The same, and when vtk_path is not null also the VTK file that
+   scmf_density_to_vtk(<that density file>, vtk_path) would make of it
+   (vtk.c:42-76), written by the background thread from the same snapshot. */
+int scmf_write_outputs_async(scmf_sys* sys, const char* dir, int32_t step, const char* vtk_path);
 /* Wait until every queued file is written; returns the first error since the
    last report.  scmf_destroy also finishes the queue. */
 int scmf_output_wait(scmf_sys* sys);
@@ -232,6 +236,12 @@ int scmf_write_density_host(const char* path, const int32_t n[3], const double L
                             const uint32_t* counts, const float* omega, double field_scaling);
 /* vtk.c:4-81: density_*.dat -> STRUCTURED_POINTS VTK of phi_0 - phi_1. */
 int scmf_density_to_vtk(const char* density_path, const char* vtk_path);
+/* The VTK file scmf_density_to_vtk(density_name, vtk_path) makes of the file
+   scmf_write_density_host writes from the same counts and omega, made from
+   them directly (no density file is read). */
+int scmf_write_vtk_host(const char* vtk_path, const char* density_name, const int32_t n[3],
+                        const double L[3], int32_t T, const uint32_t* counts, const float* omega,
+                        double field_scaling);
 
 /* ---- batches: many independent systems advanced together -------------------
  * No counterpart in the reference (one system per process).  Each step runs

@@ -1642,7 +1642,8 @@ struct AsyncOut {
   };
   struct Job {
     int slot;
-    std::string path;
+    std::string path;  // density_<step>.dat
+    std::string vtk;   // the VTK file of it, or empty
   };
   Slot slot[SLOTS];
   int device = 0;
@@ -1678,7 +1679,10 @@ struct AsyncOut {
       for (int64_t i = 0; i < TC; ++i) wide[i] = (sl.h_counts[i >> 1] >> ((i & 1) * 16)) & 0xFFFFu;
       counts = wide.data();
     }
-    return write_density_file(j.path.c_str(), n, L, T, n_cells, counts, sl.h_omega, fs);
+    std::string err = write_density_file(j.path.c_str(), n, L, T, n_cells, counts, sl.h_omega, fs);
+    if (err.empty() && !j.vtk.empty())
+      err = write_vtk_file(j.vtk.c_str(), j.path.c_str(), n, L, T, n_cells, counts, sl.h_omega, fs);
+    return err;
   }
 
   void run() {
@@ -2236,6 +2240,10 @@ int scmf_sizes(scmf_sys* s, int64_t* n_beads, int64_t* n_chains, int64_t* n_cell
 }
 
 int scmf_write_density_async(scmf_sys* s, const char* dir, int32_t step) {
+  return scmf_write_outputs_async(s, dir, step, nullptr);
+}
+
+int scmf_write_outputs_async(scmf_sys* s, const char* dir, int32_t step, const char* vtk_path) {
   if (!s) return fail(SCMF_E_ARG, "null handle");
   if (!s->out) {
     int rc = make_async_out(s);
@@ -2271,7 +2279,7 @@ int scmf_write_density_async(scmf_sys* s, const char* dir, int32_t step) {
   {
     std::lock_guard<std::mutex> lk(o->mu);
     if (e == hipSuccess)
-      o->q.push_back({k, path});
+      o->q.push_back({k, path, vtk_path ? std::string(vtk_path) : std::string()});
     else
       sl.busy = false;
   }
@@ -2445,6 +2453,17 @@ int scmf_write_density_host(const char* path, const int32_t n[3], const double L
   if (!path || !n || !L || !counts || !omega || T <= 0) return fail(SCMF_E_ARG, "null argument");
   const int64_t C = (int64_t)n[0] * n[1] * n[2];
   std::string e = write_density_file(path, n, L, T, C, counts, omega, field_scaling);
+  if (!e.empty()) return fail(SCMF_E_IO, e);
+  return SCMF_OK;
+}
+
+int scmf_write_vtk_host(const char* vtk_path, const char* density_name, const int32_t n[3],
+                        const double L[3], int32_t T, const uint32_t* counts, const float* omega,
+                        double field_scaling) {
+  if (!vtk_path || !density_name || !n || !L || !counts || !omega || T <= 0)
+    return fail(SCMF_E_ARG, "null argument");
+  const int64_t C = (int64_t)n[0] * n[1] * n[2];
+  std::string e = write_vtk_file(vtk_path, density_name, n, L, T, C, counts, omega, field_scaling);
   if (!e.empty()) return fail(SCMF_E_IO, e);
   return SCMF_OK;
 }

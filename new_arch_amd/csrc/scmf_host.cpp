@@ -234,28 +234,92 @@ std::string density_file_name(int32_t step) {
 
 namespace {
 
-// "%g " of a double, memoised by bit pattern in a direct-mapped table: the
-// values of a density file repeat (counts are small integers, omega a
+// printf text of a double, memoised by bit pattern in a direct-mapped table:
+// the values of a density file repeat (counts are small integers, omega a
 // function of them), and glibc's %g costs ~250 ns.  A hit returns the text
-// snprintf produced for the same bits, so the output is unchanged.
+// snprintf produced for the same bits, so the output is unchanged.  Text
+// longer than a table entry is formatted every time.
 struct GFormat {
   static constexpr int BITS = 13;
+  const char* fmt;
   uint64_t key[1 << BITS];
   uint8_t len[1 << BITS];  // 0: empty
   char text[1 << BITS][31];
-  GFormat() { memset(len, 0, sizeof len); }
+  explicit GFormat(const char* f = "%g ") : fmt(f) { memset(len, 0, sizeof len); }
   void append(std::string& out, double v) {
     uint64_t b;
     memcpy(&b, &v, 8);
     const uint32_t h = (uint32_t)((b * 0x9E3779B97F4A7C15ull) >> (64 - BITS));
     if (len[h] == 0 || key[h] != b) {
-      const int k = snprintf(text[h], sizeof text[h], "%g ", v);
+      const int k = snprintf(text[h], sizeof text[h], fmt, v);
+      if (k <= 0 || k >= (int)sizeof text[h]) {  // too long for the table
+        len[h] = 0;
+        std::vector<char> big((size_t)(k > 0 ? k : 0) + 1);
+        snprintf(big.data(), big.size(), fmt, v);
+        out.append(big.data(), big.size() - 1);
+        return;
+      }
       key[h] = b;
       len[h] = (uint8_t)k;
     }
     out.append(text[h], len[h]);
   }
 };
+
+// The double that the "%g" text of v reads back as (what a reader of the
+// density file gets), memoised the same way.
+struct GReadBack {
+  static constexpr int BITS = 13;
+  uint64_t key[1 << BITS];
+  double val[1 << BITS];
+  uint8_t used[1 << BITS];
+  GReadBack() { memset(used, 0, sizeof used); }
+  double get(double v) {
+    uint64_t b;
+    memcpy(&b, &v, 8);
+    const uint32_t h = (uint32_t)((b * 0x9E3779B97F4A7C15ull) >> (64 - BITS));
+    if (!used[h] || key[h] != b) {
+      char t[48];
+      snprintf(t, sizeof t, "%g", v);
+      val[h] = strtod(t, nullptr);
+      key[h] = b;
+      used[h] = 1;
+    }
+    return val[h];
+  }
+};
+
+// Formats blocks [0, blocks) of text with up to eight threads and writes
+// them in order: fmt(block, out_string, worker).
+template <class F>
+void write_blocks(FILE* f, int64_t blocks, F fmt) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int W = (int)std::min<int64_t>(std::min(8u, hw), std::max<int64_t>(blocks, 1));
+  std::vector<std::string> part((size_t)W);
+  for (int64_t b0 = 0; b0 < blocks; b0 += W) {
+    const int nb = (int)std::min<int64_t>(W, blocks - b0);
+    auto run = [&](int w) { fmt(b0 + w, part[(size_t)w], w); };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nb; ++w) th.emplace_back(run, w);
+    run(0);
+    for (auto& t : th) t.join();
+    for (int w = 0; w < nb; ++w) fwrite(part[(size_t)w].data(), 1, part[(size_t)w].size(), f);
+  }
+}
+
+// Closes f (written as tmp) and renames it to path.
+std::string finish_file(FILE* f, const std::string& tmp, const char* path) {
+  const bool bad = ferror(f) != 0;
+  if (fclose(f) != 0 || bad) {
+    remove(tmp.c_str());
+    return std::string("write error on ") + tmp;
+  }
+  if (rename(tmp.c_str(), path) != 0) {
+    remove(tmp.c_str());
+    return std::string("cannot rename ") + tmp + " to " + path;
+  }
+  return "";
+}
 
 // Rows [c0, c1) of a density file: T scaled counts, then T omega values,
 // each printed "%g ".
@@ -273,8 +337,9 @@ void format_density_rows(std::string& out, GFormat& g, int64_t c0, int64_t c1, i
 }  // namespace
 
 // io_out_density's text format (io.cpp:4-56), formatted by up to eight
-// threads (each with its own GFormat) in blocks of 65,536 cells and written in cell order to PATH.tmp,
-// then renamed to PATH (a reader never sees a partial file).
+// threads (each with its own GFormat) in blocks of 65,536 cells and written
+// in cell order to PATH.tmp, then renamed to PATH (a reader never sees a
+// partial file).
 std::string write_density_file(const char* path, const int32_t n[3], const double L[3], int T,
                                int64_t n_cells, const uint32_t* counts, const float* omega,
                                double field_scaling) {
@@ -284,35 +349,67 @@ std::string write_density_file(const char* path, const int32_t n[3], const doubl
   fprintf(f, "Nx=%d, Ny=%d, Nz=%d\n", n[0], n[1], n[2]);
   fprintf(f, "dx=%g, dy=%g, dz=%g\n", L[0] / n[0], L[1] / n[1], L[2] / n[2]);
   constexpr int64_t ROWS = 1 << 16;
-  const int64_t blocks = (n_cells + ROWS - 1) / ROWS;
-  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-  const int W = (int)std::min<int64_t>(std::min(8u, hw), std::max<int64_t>(blocks, 1));
-  std::vector<std::string> part((size_t)W);
-  std::vector<std::unique_ptr<GFormat>> cache((size_t)W);
+  std::vector<std::unique_ptr<GFormat>> cache(8);
   for (auto& g : cache) g.reset(new GFormat());
-  for (int64_t b0 = 0; b0 < blocks; b0 += W) {
-    const int nb = (int)std::min<int64_t>(W, blocks - b0);
-    auto fmt = [&](int w) {
-      const int64_t c0 = (b0 + w) * ROWS;
-      format_density_rows(part[(size_t)w], *cache[(size_t)w], c0, std::min(c0 + ROWS, n_cells), T,
-                          n_cells, counts, omega, field_scaling);
-    };
-    std::vector<std::thread> th;
-    for (int w = 1; w < nb; ++w) th.emplace_back(fmt, w);
-    fmt(0);
-    for (auto& t : th) t.join();
-    for (int w = 0; w < nb; ++w) fwrite(part[(size_t)w].data(), 1, part[(size_t)w].size(), f);
+  write_blocks(f, (n_cells + ROWS - 1) / ROWS, [&](int64_t blk, std::string& out, int w) {
+    const int64_t c0 = blk * ROWS;
+    format_density_rows(out, *cache[(size_t)w], c0, std::min(c0 + ROWS, n_cells), T, n_cells,
+                        counts, omega, field_scaling);
+  });
+  return finish_file(f, tmp, path);
+}
+
+// The VTK file density_to_vtk makes of that density file, straight from the
+// counts and omega: every value goes through its "%g" text and back, as the
+// converter reads it, and each point is the first column minus the second
+// (type 0 minus type 1, or minus omega_0 when T = 1).  density_name is the
+// name the converter's comment line carries.
+std::string write_vtk_file(const char* path, const char* density_name, const int32_t n[3],
+                           const double L[3], int T, int64_t n_cells, const uint32_t* counts,
+                           const float* omega, double field_scaling) {
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* f = fopen(tmp.c_str(), "w");
+  if (!f) return std::string("cannot open ") + tmp + " for writing";
+  double d[3];
+  for (int a = 0; a < 3; ++a) {
+    char t[48];
+    snprintf(t, sizeof t, "%g", L[a] / n[a]);
+    d[a] = strtod(t, nullptr);
   }
-  const bool bad = ferror(f) != 0;
-  if (fclose(f) != 0 || bad) {
-    remove(tmp.c_str());
-    return std::string("write error on ") + tmp;
+  fprintf(f, "# vtk DataFile Version 3.0\n");
+  fprintf(f, "# comment: converted from file %s\n", density_name);
+  fprintf(f, "ASCII\n");
+  fprintf(f, "DATASET STRUCTURED_POINTS\n");
+  fprintf(f, "DIMENSIONS %d %d %d\n", n[0], n[1], n[2]);
+  fprintf(f, "ORIGIN 0.000000 0.000000 0.000000\n");
+  fprintf(f, "SPACING %.6lf %.6lf %.6lf\n\n", d[0], d[1], d[2]);
+  fprintf(f, "POINT_DATA %ld\n", (long)n[0] * n[1] * n[2]);
+  fprintf(f, "SCALARS scalars float\n");
+  fprintf(f, "LOOKUP_TABLE default\n\n");
+  const int64_t rows = (int64_t)n[1] * n[2];
+  const int64_t per = std::max<int64_t>(1, (1 << 16) / n[0]);
+  std::vector<std::unique_ptr<GFormat>> fmt(8);
+  std::vector<std::unique_ptr<GReadBack>> rb(8);
+  for (int w = 0; w < 8; ++w) {
+    fmt[(size_t)w].reset(new GFormat("%7.4lf "));
+    rb[(size_t)w].reset(new GReadBack());
   }
-  if (rename(tmp.c_str(), path) != 0) {
-    remove(tmp.c_str());
-    return std::string("cannot rename ") + tmp + " to " + path;
-  }
-  return "";
+  write_blocks(f, (rows + per - 1) / per, [&](int64_t blk, std::string& out, int w) {
+    out.clear();
+    GFormat& g = *fmt[(size_t)w];
+    GReadBack& r = *rb[(size_t)w];
+    const int64_t r1 = std::min(rows, (blk + 1) * per);
+    for (int64_t row = blk * per; row < r1; ++row) {
+      for (int i = 0; i < n[0]; ++i) {
+        const int64_t c = row * n[0] + i;
+        const double a = r.get(counts[c] * field_scaling);
+        const double b = T >= 2 ? r.get(counts[c + n_cells] * field_scaling) : r.get((double)omega[c]);
+        g.append(out, a - b);
+      }
+      out.push_back('\n');
+    }
+  });
+  return finish_file(f, tmp, path);
 }
 
 std::string write_coord_file(const char* path, const float* xyz, int64_t n_beads) {

@@ -42,6 +42,12 @@ std::string write_density_file(const char* path, const int32_t n[3], const doubl
                                int64_t n_cells, const uint32_t* counts, const float* omega,
                                double field_scaling);
 
+// The VTK file density_to_vtk(density_name, path) makes of the density file
+// write_density_file writes from the same data.
+std::string write_vtk_file(const char* path, const char* density_name, const int32_t n[3],
+                           const double L[3], int T, int64_t n_cells, const uint32_t* counts,
+                           const float* omega, double field_scaling);
+
 // Write coord.dat (io.cpp:101-107).
 std::string write_coord_file(const char* path, const float* xyz, int64_t n_beads);
 

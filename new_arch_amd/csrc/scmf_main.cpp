@@ -11,6 +11,8 @@
 //   -checkpoint=PATH   exact checkpoint after the last step
 //   -restart=PATH      continue from such a checkpoint (instead of the
 //                      initial gridding; the run starts after its step)
+//   -vtk=1             also write density_<step>.vtk (vtk.c's conversion of
+//                      that density file) beside every dump
 // density_<step>.dat files are written by a background thread from device
 // snapshots while the steps go on; the driver waits for them after the loop.
 #include <hip/hip_runtime.h>
@@ -87,6 +89,7 @@ int main(int argc, char** argv) {
   const char* sd = flag_value(argc, argv, "seed");
   const char* restart = flag_value(argc, argv, "restart");        // exact checkpoint to continue
   const char* checkpoint = flag_value(argc, argv, "checkpoint");  // written after the last step
+  const bool vtk = flag_int(argc, argv, "vtk", 0) != 0;            // density_<step>.vtk beside each dump
 
   scmf_opts o{};
   o.seed = sd ? strtoull(sd, nullptr, 10) : (uint64_t)time(nullptr);
@@ -131,7 +134,12 @@ int main(int argc, char** argv) {
       if ((rc = scmf_stats(sys, &st))) return die("stats", rc);
       printf("step %d end to end dis %g\n", step, st.re2);
       if (st.overflow) fprintf(stderr, "scmf_run: warning: uint16 density count overflow\n");
-      if ((rc = scmf_write_density_async(sys, outdir ? outdir : "", step)))
+      std::string vtk_path;
+      if (vtk) {
+        vtk_path = std::string(outdir ? outdir : ".") + "/density_" + std::to_string(step) + ".vtk";
+      }
+      if ((rc = scmf_write_outputs_async(sys, outdir ? outdir : "", step,
+                                         vtk ? vtk_path.c_str() : nullptr)))
         return die("io_out_density", rc);
     }
   }

@@ -53,6 +53,9 @@ def lib():
     L.scmf_write_density.argtypes = [vp, C.c_char_p, C.c_int32]
     L.scmf_write_coord.argtypes = [vp, C.c_char_p]
     L.scmf_write_density_async.argtypes = [vp, C.c_char_p, C.c_int32]
+    L.scmf_write_outputs_async.argtypes = [vp, C.c_char_p, C.c_int32, C.c_char_p]
+    L.scmf_write_vtk_host.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_double), C.c_int32, vp, vp, C.c_double]
     L.scmf_output_wait.argtypes = [vp]
     L.scmf_checkpoint_write.argtypes = [vp, C.c_char_p, C.c_int64]
     L.scmf_checkpoint_read.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
@@ -249,6 +252,12 @@ class System:
         """Queue density_<step>.dat from a device snapshot of the current
         state; the file is written by a background thread."""
         _check(lib().scmf_write_density_async(self._h, str(directory).encode(), int(step)))
+
+    def write_outputs_async(self, step, directory="", vtk_path=None):
+        """write_density_async, plus the VTK file of that density file at
+        vtk_path when given."""
+        _check(lib().scmf_write_outputs_async(self._h, str(directory).encode(), int(step),
+                                              None if vtk_path is None else str(vtk_path).encode()))
 
     def output_wait(self):
         _check(lib().scmf_output_wait(self._h))

@@ -113,6 +113,33 @@ def test_density_file_over_several_format_blocks(tmp_path):
     assert not os.path.exists(str(out) + ".tmp")
 
 
+@pytest.mark.parametrize("T", [1, 2, 3])
+def test_vtk_from_counts_equals_converter(tmp_path, T):
+    """The VTK file written straight from counts and omega (the background
+    writer's path) is byte-equal to converting the density file written from
+    them: every value read back through its "%g" text."""
+    nx, ny, nz = 40, 30, 60                   # 72,000 cells: more than one block
+    cells = nx * ny * nz
+    rng = np.random.default_rng(5 + T)
+    counts = rng.integers(0, 40, size=T * cells).astype(np.uint32)
+    counts[:7] = [0, 65535, 123456, 1, 2, 3, 99999]
+    omega = (rng.standard_normal(T * cells) * 7).astype(np.float32)
+    n = (C.c_int32 * 3)(nx, ny, nz)
+    L = (C.c_double * 3)(5.0, 3.3, 7.77)
+    fs = 13824 / 409600
+    dens = str(tmp_path / "density_12.dat")
+    lib = scmf.lib()
+    assert lib.scmf_write_density_host(dens.encode(), n, L, T, counts.ctypes.data,
+                                       omega.ctypes.data, C.c_double(fs)) == 0
+    conv = tmp_path / "conv.vtk"
+    assert lib.scmf_density_to_vtk(dens.encode(), str(conv).encode()) == 0
+    direct = tmp_path / "direct.vtk"
+    assert lib.scmf_write_vtk_host(str(direct).encode(), dens.encode(), n, L, T, counts.ctypes.data,
+                                   omega.ctypes.data, C.c_double(fs)) == 0
+    assert direct.read_bytes() == conv.read_bytes()
+    assert not os.path.exists(str(direct) + ".tmp")
+
+
 def test_density_file_names():
     L = scmf.lib()
     # the name rule of io.cpp:16-19 is exercised through the host writer path

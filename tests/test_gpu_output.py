@@ -56,3 +56,25 @@ def test_async_errors_and_close(tmp_path):
         text = (tmp_path / f"density_{k}.dat").read_text()
         assert text.startswith("Nx=8, Ny=8, Nz=8\n")
         assert text.count("\n") == 2 + 8 * 8 * 8
+
+
+def test_async_vtk_equals_converter(tmp_path):
+    """write_outputs_async with a VTK path: the VTK file equals converting the
+    density file written with it (and that file equals the synchronous one)."""
+    g = scmf.System(W.tiny(n_chains=900, N=16, grid=(12, 10, 8), L=(2.4, 2.0, 1.6)),
+                    make_opts(seed=43, density_bits=16))
+    try:
+        g.grid_density()
+        g.update_omega()
+        g.write_outputs_async(5, tmp_path, vtk_path=tmp_path / "a.vtk")
+        sync_dir = tmp_path / "sync"
+        sync_dir.mkdir()
+        g.write_density(5, sync_dir)
+        g.step(1)
+        g.output_wait()
+        dens = tmp_path / "density_5.dat"
+        assert dens.read_bytes() == (sync_dir / "density_5.dat").read_bytes()
+        assert scmf.lib().scmf_density_to_vtk(str(dens).encode(), str(tmp_path / "c.vtk").encode()) == 0
+        assert (tmp_path / "a.vtk").read_bytes() == (tmp_path / "c.vtk").read_bytes()
+    finally:
+        g.close()

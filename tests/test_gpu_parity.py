@@ -375,8 +375,16 @@ def test_files_and_driver(golden, tmp_path):
     for f in ("configuration.dat", "polymer.dat"):
         (tmp_path / f).write_text(open(os.path.join(inp, f)).read())
     exe = os.path.join(ROOT, "new_arch_amd", "lib", "scmf_run")
-    r = subprocess.run([exe, "-Steps_N=20", "-Check_T=10", "-seed=5"], cwd=tmp_path,
+    r = subprocess.run([exe, "-Steps_N=20", "-Check_T=10", "-seed=5", "-vtk=1"], cwd=tmp_path,
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "step 10 end to end dis" in r.stdout and "time =" in r.stdout
     assert (tmp_path / "density_20.dat").exists()
+    # -vtk=1: density_20.vtk is vtk.c's conversion of density_20.dat
+    conv = tmp_path / "conv.vtk"
+    assert scmf.lib().scmf_density_to_vtk(str(tmp_path / "density_20.dat").encode(),
+                                          str(conv).encode()) == 0
+    a = (tmp_path / "density_20.vtk").read_text().split("\n")
+    b = conv.read_text().split("\n")
+    assert a[1] == "# comment: converted from file density_20.dat"
+    assert a[:1] + a[2:] == b[:1] + b[2:]
