@@ -176,8 +176,12 @@ def served(n_jobs, rate, seed=0, steps_per_tick=25):
     t0 = sv.clock()
     submitted, live, lat, moves = 0, {}, [], 0
     t_first = None
+    next_note = 30.0
     while len(lat) < n_jobs:
         now = sv.clock() - t0
+        if now >= next_note:  # progress of a long run
+            log(f"served: {len(lat)} of {n_jobs} jobs done, {submitted} submitted, {now:.0f} s")
+            next_note += 30.0
         while submitted < n_jobs and arrivals[submitted] <= now:
             spec, sweeps = js[submitted]
             jid = sv.submit(spec, make_opts(seed=10_000 + submitted, system_id=submitted,
