@@ -76,6 +76,8 @@ struct KParams {
   uint32_t* ctr;                 // [0] stream sweep word of the next sweep
   uint32_t* cells;               // packed cell of every stored bead, written by the sweep
                                  // (x | y << 10 | z << 20), or null
+  uint8_t* sched;                // per slot: the sweep's move schedule made by k_schedule
+  int sched_stride;              // bytes per slot record
 };
 
 // Accepted moves are counted in this many separate words, chosen by

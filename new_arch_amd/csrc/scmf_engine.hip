@@ -371,6 +371,12 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
 // entry of each level); list[N] (moves by level).
 // (At most 96 VGPRs, i.e. five waves per SIMD: the LDS tiles allow about
 // as many, and the move is issue-bound, so occupancy is worth the spills.)
+// Byte k of the move-schedule record of a slot (records interleaved by
+// groups of 64 slots; made by k_schedule below).
+__device__ __forceinline__ int64_t sched_index(int64_t slot, int stride, int k) {
+  return ((slot >> 6) * (int64_t)stride + k) * 64 + (slot & 63);
+}
+
 template <int L, typename I>
 __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict__ Ps,
                                                     const BlockRef* __restrict__ bt, int nmax) {
@@ -389,10 +395,14 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
   float* sy = lds + nmax * G;
   float* sz = lds + 2 * nmax * G;
   I* book = (I*)(lds + 3 * nmax * G);
-  const int per = 4 * nmax + 4;
+  // A schedule made by k_schedule (single systems) needs start and list
+  // only; otherwise the chain's lanes make it here from lev and W.
+  const SCMF_GLOBAL uint8_t* __restrict__ sched = gptr((const uint8_t*)kp.sched);
+  const bool pre = sched != nullptr;
+  const int per = pre ? 2 * nmax + 4 : 4 * nmax + 4;
   I* lev = book + slot * per;
   I* W = lev + nmax;
-  I* start = W + nmax + 2;
+  I* start = pre ? lev : W + nmax + 2;
   I* list = start + nmax + 2;
   const int64_t chain = (int64_t)br.local * G + slot;
   const bool active = chain < kp.n_chains;
@@ -411,6 +421,16 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
     sy[b * G + slot] = y[off + b];
     sz[b * G + slot] = z[off + b];
   }
+  int maxl = 0;
+  if (pre) {
+    if (active) {
+      const int st = kp.sched_stride;
+      maxl = sched[sched_index(chain, st, 0)];
+      for (int k = r; k <= length + 1; k += L) start[k] = (I)sched[sched_index(chain, st, 1 + k)];
+      for (int k = r; k < length; k += L) list[k] = (I)sched[sched_index(chain, st, length + 3 + k)];
+    }
+    __syncthreads();
+  } else {
   for (int i = r; i < length; i += L) {
     const u32x4 pick = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i, SLOT_PICK);
     int ib = (int)(u01_53(pick.v[0], pick.v[1]) * length);
@@ -459,7 +479,8 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
     W[0] = (I)maxl;
   }
   __syncthreads();
-  const int maxl = active ? (int)W[0] : 0;
+  maxl = active ? (int)W[0] : 0;
+  }
   uint32_t acc = 0;
   const LdsChain<G> c{sx + slot, sy + slot, sz + slot};
   int l = 1, j = 0;
@@ -504,6 +525,86 @@ template <int L, typename I>
 size_t sched_lds_bytes(int nmax) {
   constexpr int G = 64 / L;
   return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (4 * nmax + 4) * sizeof(I);
+}
+
+// The same kernel reading a schedule made by k_schedule.
+size_t sched_pre_lds_bytes(int L, int nmax) {
+  const int G = 64 / L;
+  return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (2 * nmax + 4);
+}
+
+// The move schedule of every chain for the coming sweep, one lane per slot
+// (k_sweep_sched makes the same schedule with one lane of each chain's L,
+// the others idle).  The picks depend on the streams only, so the schedule
+// does not wait for positions.  Record of a slot (uint8, chains of at most
+// 255 beads): entry 0 the number of levels, entries 1 .. N+2 start[0 .. N+1]
+// (the first list entry of each level), entries N+3 .. 2N+2 the move
+// indices by level.  Records are interleaved by groups of 64 slots: entry k
+// of slot s is byte ((s / 64) * stride + k) * 64 + s % 64, so the lanes of a
+// wave store one entry each with one contiguous store.  The lane arrays live
+// in LDS interleaved the same way (entry k of lane t at k * 64 + t).
+constexpr int SCHED_LANES = 64;
+size_t schedule_lds_bytes(int nmax) { return (size_t)(3 * nmax + 4) * SCHED_LANES; }
+
+__global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restrict__ Ps, int nmax) {
+  extern __shared__ uint8_t sh[];
+  KPC& kp = param(Ps, 0);
+  const int t = threadIdx.x;
+  const int64_t chain = (int64_t)blockIdx.x * SCHED_LANES + t;
+  if (chain >= kp.n_chains) return;
+  const uint32_t sweep = ((const uint32_t*)kp.ctr)[0];
+  const int p = arch_of_chain(kp, chain);
+  const int length = kp.len[p];
+  const uint32_t id = gptr((const uint32_t*)kp.slot_chain)[chain];
+  uint8_t* const lev = sh + t;                           // [nmax]
+  uint8_t* const W = lev + nmax * SCHED_LANES;           // [nmax + 2], W[b + 1] = level of bead b
+  uint8_t* const start = W + (nmax + 2) * SCHED_LANES;   // [nmax + 2]
+  auto at = [](uint8_t* a, int k) -> uint8_t& { return a[k * SCHED_LANES]; };
+  SCMF_GLOBAL uint8_t* __restrict__ rec = gptr(kp.sched);
+  const int stride = kp.sched_stride;
+  auto pick = [&](int i) {
+    const u32x4 w = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i, SLOT_PICK);
+    const int ib = (int)(u01_53(w.v[0], w.v[1]) * length);
+    return ib == length ? length - 1 : ib;
+  };
+  for (int k = 0; k <= length + 1; ++k) {
+    at(W, k) = 0;
+    at(start, k) = 0;
+  }
+  // levels, the next move's pick drawn while this move's LDS reads are in flight
+  const bool linear = kp.linear[p] != 0;
+  int maxl = 0;
+  int b = length > 0 ? pick(0) : 0;
+  for (int i = 0; i < length; ++i) {
+    const int cur = b;
+    int l;
+    if (linear) {
+      const int w0 = at(W, cur), w1 = at(W, cur + 1), w2 = at(W, cur + 2);
+      if (i + 1 < length) b = pick(i + 1);
+      l = max(max(w0, w1), w2) + 1;
+    } else {
+      l = at(W, cur + 1);
+      for_neighbours(kp, p, cur, length, [&](int j) { l = max(l, (int)at(W, j + 1)); });
+      l += 1;
+      if (i + 1 < length) b = pick(i + 1);
+    }
+    at(W, cur + 1) = (uint8_t)l;
+    at(lev, i) = (uint8_t)l;
+    at(start, l + 1) = (uint8_t)(at(start, l + 1) + 1);
+    maxl = max(maxl, l);
+  }
+  for (int l = 1; l <= maxl; ++l) {
+    at(start, l + 1) = (uint8_t)(at(start, l + 1) + at(start, l));
+    at(W, l) = 0;  // fill counters from here on
+  }
+  rec[sched_index(chain, stride, 0)] = (uint8_t)maxl;
+  for (int k = 0; k <= length + 1; ++k) rec[sched_index(chain, stride, 1 + k)] = at(start, k);
+  for (int i = 0; i < length; ++i) {
+    const int l = at(lev, i);
+    const int k = at(W, l);
+    at(W, l) = (uint8_t)(k + 1);
+    rec[sched_index(chain, stride, length + 3 + at(start, l) + k)] = (uint8_t)i;
+  }
 }
 
 // Clear each system's counts (a kernel, not a memset, so it can sit in a graph).
@@ -937,6 +1038,8 @@ struct scmf_sys {
   float* z2 = nullptr;
   uint16_t* scratch = nullptr;  // window images of the tiled gridding
   uint32_t* cells = nullptr;    // packed bead cells written by the sweep (tiled systems)
+  uint8_t* sched = nullptr;     // move schedules made by k_schedule (chains of <= 255 beads)
+  bool pre_sched = false;       // the scheduled sweep shapes read them
   bool cells_fresh = false;     // cells match the positions (the last change was such a sweep)
   unsigned long long* d_fallback = nullptr;
   TileGeo geo{};
@@ -1204,7 +1307,8 @@ int choose_sweep(scmf_sys* s) {
     }
   }
   s->variant = best;
-  s->lds_bytes = best->bytes(s->nmax);
+  s->lds_bytes = s->pre_sched && best->L >= 2 ? sched_pre_lds_bytes(best->L, s->nmax)
+                                              : best->bytes(s->nmax);
   return SCMF_OK;
 }
 
@@ -1217,6 +1321,10 @@ int launch_sweep(scmf_sys* s) {
   } else {
     const int G = 64 / v->L;
     const dim3 grid((unsigned)((s->n_chains + G - 1) / G));
+    if (s->pre_sched)
+      hipLaunchKernelGGL(k_schedule, dim3((unsigned)((s->n_chains + SCHED_LANES - 1) / SCHED_LANES)),
+                         dim3(SCHED_LANES), schedule_lds_bytes(s->nmax), s->stream,
+                         (const KParams*)s->d_kp, s->nmax);
     const KParams* kpp = s->d_kp;
     void* args[] = {(void*)&kpp, (void*)&none, (void*)&s->nmax};
     HIP_TRY(hipLaunchKernel(v->fn, grid, dim3(64), args, s->lds_bytes, s->stream));
@@ -1441,7 +1549,7 @@ int free_all(scmf_sys* s) {
                   (void*)s->tstart, (void*)s->tfill, (void*)s->new_slot, (void*)s->x2,
                   (void*)s->y2, (void*)s->z2, (void*)s->scratch, (void*)s->d_fallback,
                   (void*)s->d_kp, (void*)s->d_ctr, (void*)s->fft_buf, (void*)s->d_sq_sum,
-                  (void*)s->d_sq_cnt, (void*)s->cells})
+                  (void*)s->d_sq_cnt, (void*)s->cells, (void*)s->sched})
     if (p) (void)hipFree(p);
   if (s->fft_ready) (void)hipfftDestroy(s->fft);
   for (Timed& t : s->pending) {
@@ -1494,6 +1602,7 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(SCMF_E_HIP, "hipStreamCreate failed"));
   if ((rc = upload_arch(s))) return bail(rc);
+  s->pre_sched = s->nmax <= 255;
   if ((rc = choose_sweep(s))) return bail(rc);
   const int64_t B = s->n_beads;
   const int64_t TC = (int64_t)s->T * s->n_cells;
@@ -1522,6 +1631,13 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
   if ((rc = alloc((void**)&s->d_kp, sizeof(KParams))) ||
       (rc = alloc((void**)&s->d_ctr, 2 * sizeof(uint32_t))))
     return bail(rc);
+  if (s->pre_sched) {
+    s->kp.sched_stride = 2 * s->nmax + 3;
+    if ((rc = alloc((void**)&s->sched,
+                    (size_t)s->kp.sched_stride * 64 * (size_t)((s->n_chains + 63) / 64))))
+      return bail(rc);
+    s->kp.sched = s->sched;
+  }
   s->kp.slot_chain = s->slot_chain;
   s->kp.chain_slot = s->chain_slot;
   s->kp.btype = s->btype;
@@ -2543,6 +2659,7 @@ int batch_build(scmf_batch* b) {
   for (int i = 0; i < n; ++i) {
     ps[i] = b->m[i]->kp;
     ps[i].cells = nullptr;  // the batch grids from positions
+    ps[i].sched = nullptr;  // and schedules inside the sweep
   }
   HIP_TRY(hipMemcpy(b->d_Ps, ps.data(), sizeof(KParams) * n, hipMemcpyHostToDevice));
   // sweep groups by kernel shape; members ordered by decreasing work first
@@ -2712,6 +2829,7 @@ int scmf_batch_step(scmf_batch* b, int32_t n_steps) {
       for (size_t i = 0; i < b->m.size(); ++i) {
         ps[i] = b->m[i]->kp;
         ps[i].cells = nullptr;
+        ps[i].sched = nullptr;
       }
       HIP_TRY(hipMemcpyAsync(b->d_Ps, ps.data(), sizeof(KParams) * ps.size(),
                              hipMemcpyHostToDevice, b->stream));
