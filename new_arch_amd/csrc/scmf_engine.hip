@@ -372,9 +372,9 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
 // (At most 96 VGPRs, i.e. five waves per SIMD: the LDS tiles allow about
 // as many, and the move is issue-bound, so occupancy is worth the spills.)
 // Byte k of the move-schedule record of a slot (records interleaved by
-// groups of 64 slots; made by k_schedule below).
+// groups of 8 slots; made by k_schedule below).
 __device__ __forceinline__ int64_t sched_index(int64_t slot, int stride, int k) {
-  return ((slot >> 6) * (int64_t)stride + k) * 64 + (slot & 63);
+  return ((slot >> 3) * (int64_t)stride + k) * 8 + (slot & 7);
 }
 
 template <int L, typename I>
@@ -539,10 +539,11 @@ size_t sched_pre_lds_bytes(int L, int nmax) {
 // does not wait for positions.  Record of a slot (uint8, chains of at most
 // 255 beads): entry 0 the number of levels, entries 1 .. N+2 start[0 .. N+1]
 // (the first list entry of each level), entries N+3 .. 2N+2 the move
-// indices by level.  Records are interleaved by groups of 64 slots: entry k
-// of slot s is byte ((s / 64) * stride + k) * 64 + s % 64, so the lanes of a
-// wave store one entry each with one contiguous store.  The lane arrays live
-// in LDS interleaved the same way (entry k of lane t at k * 64 + t).
+// indices by level.  Records are interleaved by groups of 8 slots (the
+// chains of one sweep wavefront at 8 lanes per chain): entry k of slot s is
+// byte ((s / 8) * stride + k) * 8 + s % 8, so a sweep wavefront reads its
+// chains' records with contiguous loads.  The lane arrays of this kernel
+// live in LDS interleaved by lane (entry k of lane t at k * 64 + t).
 constexpr int SCHED_LANES = 64;
 size_t schedule_lds_bytes(int nmax) { return (size_t)(3 * nmax + 4) * SCHED_LANES; }
 
@@ -1634,7 +1635,7 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
   if (s->pre_sched) {
     s->kp.sched_stride = 2 * s->nmax + 3;
     if ((rc = alloc((void**)&s->sched,
-                    (size_t)s->kp.sched_stride * 64 * (size_t)((s->n_chains + 63) / 64))))
+                    (size_t)s->kp.sched_stride * 8 * (size_t)((s->n_chains + 7) / 8))))
       return bail(rc);
     s->kp.sched = s->sched;
   }
