@@ -320,6 +320,12 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
     # for the gridding); the gridding reads the cells (4 B) or the positions
     # (12 B) and writes the counts
     sweep_bytes = 24 * B + 4 * TC + (4 * B if from_cells else 0)
+    # chains of at most 255 beads: k_schedule writes each chain's schedule
+    # record (2 nmax + 3 bytes) and the sweep reads it back
+    nmax = max(p.length for p in spec.polymers)
+    pre = nmax <= 255 and lanes >= 2
+    if pre:
+        sweep_bytes += 2 * (2 * nmax + 3) * spec.n_chains
     grid_bytes = (4 if from_cells else 12) * B + s * TC
     omega_bytes = s * TC + 4 * TC       # counts read, fp32 omega written
     tiled = spec.n_beads >= (1 << 23)
@@ -333,7 +339,8 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
         with open(tpath) as f:
             measured = json.load(f).get("bytes_per_launch", {})
     byte_t = "unsigned char" if spec.polymers and max(p.length for p in spec.polymers) <= 255 else "unsigned short"
-    sweep_key = f"k_sweep_sched<{lanes}, {byte_t}>" if lanes > 1 else sweep_kernel
+    sweep_keys = ([f"k_sweep_sched<{lanes}, {byte_t}>"] if lanes > 1 else [sweep_kernel]) + \
+        (["k_schedule"] if pre else [])
     omega_key = (f"k_omega4<{spec.n_types}, {bits}>" if (spec.n_cells % 4 == 0)
                  else "k_omega")
     grid_keys = [f"k_grid_tiles<{bits}, false, true>" if from_cells
@@ -366,8 +373,8 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
                    "box_Re": list(spec.L), "xn": spec.xn, "move": move,
                    "density_bits": bits, "omega": "every sweep", "parallelism": "1 GPU",
                    "lanes_per_chain": lanes},
-        "roofline": dict(roof(sweep_bytes, tm.sweep_ms, traffic_of([sweep_key])),
-                         kernel=sweep_kernel),
+        "roofline": dict(roof(sweep_bytes, tm.sweep_ms, traffic_of(sweep_keys)),
+                         kernel=("k_schedule+" if pre else "") + sweep_kernel),
         "passes": {"grid": dict(roof(grid_bytes, tm.grid_ms,
                                      traffic_of(grid_keys) if tiled else None),
                                 kernel=("k_zero_counts+k_grid_tiles(cells)+k_tiles_sum"
