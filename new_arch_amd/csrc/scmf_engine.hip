@@ -1603,7 +1603,9 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(SCMF_E_HIP, "hipStreamCreate failed"));
   if ((rc = upload_arch(s))) return bail(rc);
-  s->pre_sched = s->nmax <= 255;
+  // Schedules in their own kernel where the in-sweep scan outweighs a launch
+  // (measured: slower for systems of a few thousand chains, batches included).
+  s->pre_sched = s->nmax <= 255 && s->n_chains >= 65536;
   if ((rc = choose_sweep(s))) return bail(rc);
   const int64_t B = s->n_beads;
   const int64_t TC = (int64_t)s->T * s->n_cells;

@@ -186,3 +186,12 @@ def test_gridding_follows_every_position_change(tmp_path, L, lanes):
         check(3 if tuned >= 2 else 2)              # only the scheduled shapes store cells
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("lanes", [0, 4])
+def test_schedule_kernel_with_branched_chains(lanes):
+    """66,000 chains (homopolymers, diblocks and 4-arm stars): enough chains
+    that the move schedules come from k_schedule, whose general neighbour
+    scan the stars take.  Sweeps bit-exact against the oracle."""
+    banner(f"schedule kernel, stars, lanes {lanes}")
+    run_pair(W.mixed_arch(scale=550), sweeps=2, seed=16, density_bits=32, lanes_per_chain=lanes)
