@@ -2542,7 +2542,17 @@ void fill_summary(const scmf_desc* d, scmf_summary* out) {
   const int tt = d->n_mono_types * d->n_mono_types;
   for (int i = 0; i < tt && i < 64; ++i) out->xn[i] = d->xn[i];
   const uint64_t TC = (uint64_t)d->n_mono_types * out->n_cells;
-  const uint64_t fixed = 12ull * beads + 4ull * TC + 8ull * ((chains + BLOCK - 1) / BLOCK) + 4096;
+  int nmax = 0;
+  for (int p = 0; p < d->n_poly_types; ++p) nmax = std::max(nmax, (int)d->poly_length[p]);
+  const uint64_t B4 = (uint64_t)padded(beads);
+  // positions, monomer types, slot maps, omega, partial sums and parameters
+  uint64_t fixed = 12ull * B4 + (uint64_t)beads + 8ull * chains + 4ull * TC +
+                   8ull * ((chains + BLOCK - 1) / BLOCK) + 4096;
+  if (nmax <= 255 && chains >= 65536)  // schedule records
+    fixed += (uint64_t)(2 * nmax + 3) * 8ull * (uint64_t)((chains + 7) / 8);
+  if (beads >= ((int64_t)1 << 23))  // automatic ordering: second position buffer, cells,
+    fixed += 16ull * B4 + 16ull * chains + 8ull * TC;  // keys and maps, window images
+  fixed += 26ull << 21;  // up to 26 device allocations, each rounded to 2 MiB
   out->device_bytes_16 = fixed + 4ull * ((TC + 1) / 2);
   out->device_bytes_32 = fixed + 4ull * TC;
 }
