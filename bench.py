@@ -343,9 +343,11 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
         (["k_schedule"] if pre else [])
     omega_key = (f"k_omega4<{spec.n_types}, {bits}>" if (spec.n_cells % 4 == 0)
                  else "k_omega")
-    grid_keys = [f"k_grid_tiles<{bits}, false, true>" if from_cells
-                 else f"k_grid_tiles<{bits}, {'true' if spec_pow2(spec) else 'false'}, false>",
-                 f"k_tiles_sum<{bits}>", "k_zero_counts"]
+    pair = "true" if spec.n_types == 2 else "false"
+    grid_keys = [f"k_grid_tiles<{bits}, false, true, {pair}>" if from_cells
+                 else f"k_grid_tiles<{bits}, {'true' if spec_pow2(spec) else 'false'}, false, {pair}>",
+                 f"k_tiles_sum_pair<{bits}>" if spec.n_types == 2 else f"k_tiles_sum<{bits}>",
+                 "k_zero_counts"]
 
     def traffic_of(keys):
         if all(k in measured for k in keys):

@@ -1429,9 +1429,31 @@ void choose_tiles(scmf_sys* s) {
     bg.reg_off[r] = off;
     off += (int)((e + 1) & ~(int64_t)1);
   }
+  // (two monomer types store the window itself: win_words_padded words)
+  if (s->T == 2) off = std::max(off, 2 * bg.win_words_padded);
   bg.slot_elems = (off + 63) & ~63;
   s->geo = bg;
   s->tile_lds = (size_t)bg.win_words_padded * 4 + tables;
+}
+
+// The tiled gridding kernel: from the sweep's cells or from positions,
+// 16- or 32-bit counts, power-of-two cells (positions only), the paired
+// window layout of two monomer types.
+const void* grid_tiles_kernel(bool cells, bool b32, bool p2, bool pair) {
+  if (pair) {
+    if (cells) return b32 ? (const void*)k_grid_tiles<32, false, true, true>
+                          : (const void*)k_grid_tiles<16, false, true, true>;
+    if (b32) return p2 ? (const void*)k_grid_tiles<32, true, false, true>
+                       : (const void*)k_grid_tiles<32, false, false, true>;
+    return p2 ? (const void*)k_grid_tiles<16, true, false, true>
+              : (const void*)k_grid_tiles<16, false, false, true>;
+  }
+  if (cells) return b32 ? (const void*)k_grid_tiles<32, false, true, false>
+                        : (const void*)k_grid_tiles<16, false, true, false>;
+  if (b32) return p2 ? (const void*)k_grid_tiles<32, true, false, false>
+                     : (const void*)k_grid_tiles<32, false, false, false>;
+  return p2 ? (const void*)k_grid_tiles<16, true, false, false>
+            : (const void*)k_grid_tiles<16, false, false, false>;
 }
 
 int alloc_sort_buffers(scmf_sys* s) {
@@ -1471,18 +1493,14 @@ int alloc_sort_buffers(scmf_sys* s) {
   HIP_TRY(hipMemsetAsync(s->scratch + (size_t)s->geo.ntiles * s->geo.slot_elems, 0,
                          sizeof(uint16_t) * 64, s->stream));
   if (s->tile_lds + MONO_LDS + 128 > 65536) {
-    const void* fns[] = {(const void*)k_grid_tiles<16, false, false>,
-                         (const void*)k_grid_tiles<32, false, false>,
-                         (const void*)k_grid_tiles<16, true, false>,
-                         (const void*)k_grid_tiles<32, true, false>,
-                         (const void*)k_grid_tiles<16, false, true>,
-                         (const void*)k_grid_tiles<32, false, true>};
-    for (const void* f : fns)
+    for (int v = 0; v < 16; ++v) {
+      const void* f = grid_tiles_kernel(v & 1, v & 2, v & 4, v & 8);
       if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) !=
           hipSuccess) {
         (void)hipGetLastError();
         return fail(SCMF_E_HIP, "cannot raise the LDS limit of the tiled gridding");
       }
+    }
   }
   return SCMF_OK;
 }
@@ -1527,14 +1545,20 @@ int sort_chains(scmf_sys* s) {
 // s->cells_fresh holds) instead of binning the positions.
 int launch_tiled_grid(scmf_sys* s, hipStream_t st, const KParams* Ps, int sys, bool use_cells) {
   const dim3 tiles((unsigned)s->geo.ntiles);
-  const bool p2 = s->kp.pow2 != 0;
-  const bool b32 = s->opts.density_bits == 32;
-  auto bin = use_cells ? (b32 ? k_grid_tiles<32, false, true> : k_grid_tiles<16, false, true>)
-             : b32     ? (p2 ? k_grid_tiles<32, true, false> : k_grid_tiles<32, false, false>)
-                       : (p2 ? k_grid_tiles<16, true, false> : k_grid_tiles<16, false, false>);
+  const void* bin = grid_tiles_kernel(use_cells, s->opts.density_bits == 32, s->kp.pow2 != 0,
+                                      s->T == 2);
+  const uint32_t* tstart = s->tstart;
+  void* args[] = {(void*)&Ps, (void*)&sys, (void*)&s->geo, (void*)&tstart, (void*)&s->scratch,
+                  (void*)&s->d_fallback};
+  HIP_TRY(hipLaunchKernel(bin, tiles, dim3(GRID_THREADS), args, s->tile_lds, st));
+  if (s->T == 2) {
+    auto sum = s->opts.density_bits == 32 ? k_tiles_sum_pair<32> : k_tiles_sum_pair<16>;
+    hipLaunchKernelGGL(sum, tiles, dim3(SUM_THREADS), 0, st, Ps, sys, s->geo,
+                       (const uint32_t*)s->scratch);
+    HIP_TRY(hipGetLastError());
+    return SCMF_OK;
+  }
   auto sum = s->opts.density_bits == 32 ? k_tiles_sum<32> : k_tiles_sum<16>;
-  hipLaunchKernelGGL(bin, tiles, dim3(GRID_THREADS), s->tile_lds, st, Ps, sys, s->geo,
-                     (const uint32_t*)s->tstart, s->scratch, s->d_fallback);
   hipLaunchKernelGGL(sum, tiles, dim3(SUM_THREADS), 0, st, Ps, sys, s->geo,
                      (const uint16_t*)s->scratch);
   HIP_TRY(hipGetLastError());

@@ -241,7 +241,7 @@ __device__ __forceinline__ void add_global(SCMF_GLOBAL uint32_t* counts, uint32_
 
 // Move every uint16 half of the window that has reached 2^15 to the global
 // counts and clear it (only tiles with more than 65535 beads need this).
-template <int BITS>
+template <int BITS, bool PAIR>
 __device__ void normalise_window(uint32_t* win, const TileGeo& g, KPC& kp, int ox, int oy, int oz,
                                  SCMF_GLOBAL uint32_t* counts, uint32_t* overflow) {
   // four words per lane and test (the window is padded to a multiple of four)
@@ -257,9 +257,10 @@ __device__ void normalise_window(uint32_t* win, const TileGeo& g, KPC& kp, int o
       const uint32_t val = (v >> (16 * half)) & 0xFFFFu;
       if (val < 0x8000u) continue;
       keep &= ~(0xFFFFu << (16 * half));
+      // window cell and type of the half (see window_half)
       const int wc = 2 * w + half;
-      const int type = wc / g.win_cells;
-      const int r = wc - type * g.win_cells;
+      const int type = PAIR ? half : wc / g.win_cells;
+      const int r = PAIR ? w : wc - type * g.win_cells;
       const int wx = r % g.W[0], wy = (r / g.W[0]) % g.W[1], wz = r / (g.W[0] * g.W[1]);
       const int cx = wrap_rel(ox + wx, 0, kp.nx), cy = wrap_rel(oy + wy, 0, kp.ny),
                 cz = wrap_rel(oz + wz, 0, kp.nz);
@@ -278,19 +279,37 @@ __device__ void normalise_window(uint32_t* win, const TileGeo& g, KPC& kp, int o
 // four beads), the next four in flight while the current ones are binned.
 // With CELLS the cells the last sweep stored for the same positions are read
 // instead (4 bytes a bead, no binning).
-template <int BITS, bool POW2, bool CELLS>
+//
+// Window layout: element (type, window cell c) is uint16 half number
+// type * win_cells + c of the window; with PAIR (two monomer types) it is
+// half `type` of word c instead, so a bead's counter word is addressed by
+// its cell alone (the axis tables then hold byte offsets) and its type only
+// selects the increment, 1 or 1 << 16.
+template <bool PAIR>
+__device__ __forceinline__ uint32_t window_half(const uint32_t* win, uint32_t wc, uint32_t wcl) {
+  if (PAIR) {
+    const uint32_t type = wc >= wcl ? 1u : 0u;
+    return (win[wc - (type ? wcl : 0u)] >> (type << 4)) & 0xFFFFu;
+  }
+  return (win[wc >> 1] >> ((wc & 1u) << 4)) & 0xFFFFu;
+}
+
+template <int BITS, bool POW2, bool CELLS, bool PAIR>
 __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __restrict__ Ps, int sys,
                                                              TileGeo g,
                                                              const uint32_t* __restrict__ tile_start,
                                                              uint16_t* __restrict__ scratch,
                                                              unsigned long long* __restrict__ fallback) {
-  extern __shared__ uint32_t win[];
-  __shared__ uint8_t mono_s[MONO_LDS];
+  extern __shared__ __align__(16) uint32_t win[];
+  // types of an architecture's monomers (with PAIR, 16 x type: the shift of
+  // the increment), the first three repeated after the last so that the four
+  // beads of a lane read consecutive entries without wrapping
+  __shared__ uint8_t mono_s[MONO_LDS + 4];
   __shared__ int reg_s[27];
   KPC& kp = param(Ps, (uint32_t)sys);
   if (threadIdx.x < 27) reg_s[threadIdx.x] = g.reg_off[threadIdx.x];
   // per-axis tables after the window: cell -> offset in the window (scaled
-  // by the strides of the window box), or OUT_OF_WINDOW
+  // by the strides of the window box; with PAIR in bytes), or OUT_OF_WINDOW
   uint32_t* const tab_x = win + g.win_words_padded;
   uint32_t* const tab_y = tab_x + kp.nx;
   uint32_t* const tab_z = tab_y + kp.ny;
@@ -312,15 +331,15 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   for (int w = threadIdx.x; w < g.win_words_padded; w += GRID_THREADS) win[w] = 0u;
   for (int c = threadIdx.x; c < kp.nx; c += GRID_THREADS) {
     const int d = wrap_rel(c, ox, kp.nx);
-    tab_x[c] = d < wxm ? (uint32_t)d : OUT_OF_WINDOW;
+    tab_x[c] = d < wxm ? (uint32_t)d * (PAIR ? 4u : 1u) : OUT_OF_WINDOW;
   }
   for (int c = threadIdx.x; c < kp.ny; c += GRID_THREADS) {
     const int d = wrap_rel(c, oy, kp.ny);
-    tab_y[c] = d < wym ? (uint32_t)(d * g.W[0]) : OUT_OF_WINDOW;
+    tab_y[c] = d < wym ? (uint32_t)(d * g.W[0]) * (PAIR ? 4u : 1u) : OUT_OF_WINDOW;
   }
   for (int c = threadIdx.x; c < kp.nz; c += GRID_THREADS) {
     const int d = wrap_rel(c, oz, kp.nz);
-    tab_z[c] = d < wzm ? (uint32_t)(d * g.W[0] * g.W[1]) : OUT_OF_WINDOW;
+    tab_z[c] = d < wzm ? (uint32_t)(d * g.W[0] * g.W[1]) * (PAIR ? 4u : 1u) : OUT_OF_WINDOW;
   }
   int64_t pop = 0;
   for (int p = 0; p < kp.P; ++p)
@@ -336,7 +355,11 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   // globally) skipped.
   const bool checked = pop > 65535;
   __shared__ uint32_t redo_s;
-  if (threadIdx.x == 0) redo_s = 0u;
+  __shared__ uint32_t spare_s;  // target of the atomics of beads not counted in the window
+  if (threadIdx.x == 0) {
+    redo_s = 0u;
+    spare_s = 0u;
+  }
   const Axis ax0 = axis(kp, 0), ax1 = axis(kp, 1), ax2 = axis(kp, 2);
   uint32_t missed = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
@@ -359,8 +382,8 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
     const bool tab = len <= MONO_LDS;
     __syncthreads();
     if (tab)
-      for (int i = threadIdx.x; i < len; i += GRID_THREADS)
-        mono_s[i] = (uint8_t)kp.monotype[kp.mono_off[p] + i];
+      for (int i = threadIdx.x; i < len + 3; i += GRID_THREADS)
+        mono_s[i] = (uint8_t)(kp.monotype[kp.mono_off[p] + i % len] * (PAIR ? 16 : 1));
     __syncthreads();
     const double inv_len = kp.inv_len[p];
     // The type source is fixed per architecture; two copies of the loop keep
@@ -368,10 +391,13 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
     // would make every LDS read wait for the prefetched positions).
     auto bin_range = [&](auto tab_c) {
       constexpr bool TAB = decltype(tab_c)::value;
-      // Each lane takes four beads per pass and loads the next pass's four
-      // before binning these.  Loads are issued unconditionally (lanes past
-      // the range re-read the last group of the range), so the wait before
-      // binning counts only the current group and the next stays in flight.
+      // Each lane takes four beads per pass and keeps the loads of the next
+      // PF passes in flight while it bins these (one pass in flight per lane
+      // would cap the whole GPU near 2 TB/s at HBM latency).  Loads are
+      // issued unconditionally (lanes past the range re-read the last group
+      // of the range), so the wait before binning counts only the current
+      // group and the later ones stay in flight.
+      constexpr int PF = CELLS ? 4 : 2;
       const int64_t last = (b1 - 1) & ~(int64_t)3;
       int64_t gq = (b0 & ~(int64_t)3) + 4 * (int64_t)threadIdx.x;
       // monomer index of the lane's first bead (floor mod len), then advanced
@@ -384,82 +410,124 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
         if (m0 < 0) m0 += len;
         if (m0 >= len) m0 -= len;
       }
-      const int64_t a0 = gq < b1 ? gq : last;
-      f32x4 X = {}, Y = {}, Z = {};
-      u32v4 Q = {};
-      if constexpr (CELLS) {
-        Q = *(const SCMF_GLOBAL u32v4*)(cellw + a0);
-      } else {
-        X = *(const SCMF_GLOBAL f32x4*)(x + a0);
-        Y = *(const SCMF_GLOBAL f32x4*)(y + a0);
-        Z = *(const SCMF_GLOBAL f32x4*)(z + a0);
-      }
-      for (int64_t base = b0 & ~(int64_t)3; base < b1; base += PASS_BEADS) {
-        const int64_t gn = gq + PASS_BEADS;
-        const int64_t an = gn < b1 ? gn : last;
-        f32x4 Xn = {}, Yn = {}, Zn = {};
-        u32v4 Qn = {};
+      auto load = [&](int64_t gl, f32x4& Xl, f32x4& Yl, f32x4& Zl, u32v4& Ql) {
+        const int64_t a = gl < b1 ? gl : last;
         if constexpr (CELLS) {
-          Qn = *(const SCMF_GLOBAL u32v4*)(cellw + an);
+          Ql = *(const SCMF_GLOBAL u32v4*)(cellw + a);
         } else {
-          Xn = *(const SCMF_GLOBAL f32x4*)(x + an);
-          Yn = *(const SCMF_GLOBAL f32x4*)(y + an);
-          Zn = *(const SCMF_GLOBAL f32x4*)(z + an);
+          Xl = *(const SCMF_GLOBAL f32x4*)(x + a);
+          Yl = *(const SCMF_GLOBAL f32x4*)(y + a);
+          Zl = *(const SCMF_GLOBAL f32x4*)(z + a);
         }
+      };
+      f32x4 Xr[PF] = {}, Yr[PF] = {}, Zr[PF] = {};
+      u32v4 Qr[PF] = {};
+      // (issued in pass order: loads complete in order, and the wait for
+      // pass k must not cover the later ones)
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+        load(gq + (int64_t)k * PASS_BEADS, Xr[k], Yr[k], Zr[k], Qr[k]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      // PF passes per iteration: slot k of the ring always holds pass k
+      // (mod PF), so no register of a load in flight is ever copied (a copy
+      // would wait for the load).
+      int64_t base = b0 & ~(int64_t)3;
+      while (base < b1) {
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+      if (base < b1) {
+        const f32x4 X = Xr[k], Y = Yr[k], Z = Zr[k];
+        const u32v4 Q = Qr[k];
+        load(gq + (int64_t)PF * PASS_BEADS, Xr[k], Yr[k], Zr[k], Qr[k]);
         if (gq < b1) {
-          int m = m0;
           const float px[4] = {X.x, X.y, X.z, X.w}, py[4] = {Y.x, Y.y, Y.z, Y.w},
                       pz[4] = {Z.x, Z.y, Z.z, Z.w};
           const uint32_t pq[4] = {Q.x, Q.y, Q.z, Q.w};
           // beads e in [elo, ehi) of the four belong to the range
           const int elo = gq < b0 ? (int)(b0 - gq) : 0;
           const int ehi = b1 - gq < 4 ? (int)(b1 - gq) : 4;
+          // cell of bead e.  A bead outside the range decodes to a cell of
+          // the box too (so its table reads stay in bounds): its cell word is
+          // another bead's or the padding's, which is zeroed when allocated,
+          // and the fold maps any coordinate, even a non-finite one, into
+          // the box.
+          auto decode = [&](int e, int* xi, int* yi, int* zi) {
+            if constexpr (CELLS) {
+              *xi = (int)(pq[e] & 1023u);
+              *yi = (int)((pq[e] >> 10) & 1023u);
+              *zi = (int)(pq[e] >> 20);
+            } else {
+              const bool quick = POW2 && (fold_pow2(px[e], ax0, xi) & fold_pow2(py[e], ax1, yi) &
+                                          fold_pow2(pz[e], ax2, zi));
+              if (!quick) fold3_general(px[e], py[e], pz[e], ax0, ax1, ax2, xi, yi, zi);
+            }
+          };
+          // The LDS work of the four beads is issued in two batches (the
+          // table and type reads, then the counter atomics), so each lane
+          // waits for LDS twice per pass rather than twice per bead.
+          uint32_t sp[4], ty[4];
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
-            if (e >= elo && e < ehi) {
-              const uint32_t ty_ = TAB ? (uint32_t)mono_s[m] : (uint32_t)btype[gq + e];
-              int xi, yi, zi;
-              if constexpr (CELLS) {
-                xi = (int)(pq[e] & 1023u);
-                yi = (int)((pq[e] >> 10) & 1023u);
-                zi = (int)(pq[e] >> 20);
-              } else {
-                const bool quick = POW2 && (fold_pow2(px[e], ax0, &xi) &
-                                            fold_pow2(py[e], ax1, &yi) & fold_pow2(pz[e], ax2, &zi));
-                if (!quick) fold3_general(px[e], py[e], pz[e], ax0, ax1, ax2, &xi, &yi, &zi);
-              }
-              const uint32_t sp = tab_x[xi] + tab_y[yi] + tab_z[zi];
-              if (sp < OUT_OF_WINDOW) {
-                const uint32_t wc = __umul24(ty_, (uint32_t)g.win_cells) + sp;
-                if (track)
-                  seen |= atomicAdd(&win[wc >> 1], 1u << ((wc & 1u) << 4));
-                else
-                  atomicAdd(&win[wc >> 1], 1u << ((wc & 1u) << 4));
-              } else if (!safe) {
+            int xi, yi, zi;
+            decode(e, &xi, &yi, &zi);
+            sp[e] = tab_x[xi] + tab_y[yi] + tab_z[zi];
+            if (TAB)
+              ty[e] = (uint32_t)mono_s[m0 + e];
+            else
+              ty[e] = e >= elo && e < ehi ? (uint32_t)btype[gq + e] * (PAIR ? 16u : 1u) : 0u;
+          }
+          // (keeps the compiler from using each bead's reads before the
+          // next bead's are issued)
+          __builtin_amdgcn_sched_barrier(0);
+          // Every lane issues all four atomics: a bead that is not counted
+          // here adds 0 to a spare word, so no branch (and no wait the
+          // compiler would place before one) separates them.
+          uint32_t got[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const bool here = e >= elo && e < ehi && sp[e] < OUT_OF_WINDOW;
+            uint32_t* word;
+            uint32_t inc;
+            if (PAIR) {
+              word = (uint32_t*)((char*)win + sp[e]);
+              inc = 1u << ty[e];
+            } else {
+              const uint32_t wc = __umul24(ty[e], (uint32_t)g.win_cells) + sp[e];
+              word = &win[wc >> 1];
+              inc = 1u << ((wc & 1u) << 4);
+            }
+            got[e] = atomicAdd(here ? word : &spare_s, here ? inc : 0u);
+          }
+          // (the returned words matter only when tracked)
+          const uint32_t got4 = (got[0] | got[1]) | (got[2] | got[3]);
+          seen |= track ? got4 : 0u;
+          if (!safe) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              if (e >= elo && e < ehi && sp[e] >= OUT_OF_WINDOW) {
+                int xi, yi, zi;
+                decode(e, &xi, &yi, &zi);
                 ++missed;
+                const uint32_t type = PAIR ? ty[e] >> 4 : ty[e];
                 add_global<BITS>(counts, overflow,
-                                 ty_ * kp.n_cells + (uint32_t)(xi + kp.nx * (yi + kp.ny * zi)), 1u);
+                                 type * kp.n_cells + (uint32_t)(xi + kp.nx * (yi + kp.ny * zi)), 1u);
               }
             }
-            if (++m == len) m = 0;
           }
         }
-        if constexpr (CELLS) {
-          Q = Qn;
-        } else {
-          X = Xn;
-          Y = Yn;
-          Z = Zn;
-        }
-        gq = gn;
+        gq += PASS_BEADS;
         m0 += step_m;
         if (m0 >= len) m0 -= len;
         if (safe && (since += PASS_BEADS) >= NORM_BEADS) {
           since = 0;
           __syncthreads();
-          normalise_window<BITS>(win, g, kp, ox, oy, oz, counts, overflow);
+          normalise_window<BITS, PAIR>(win, g, kp, ox, oy, oz, counts, overflow);
           __syncthreads();
         }
+        base += PASS_BEADS;
+      }
+      }
       }
     };
     if (tab)
@@ -476,6 +544,13 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   for (int off = 32; off > 0; off >>= 1) missed += __shfl_xor(missed, off);
   if ((threadIdx.x & 63) == 0 && missed) atomicAdd(fallback, (unsigned long long)missed);
   __syncthreads();
+  if (PAIR) {
+    // the window verbatim, 16 bytes a lane (images are 128-byte aligned)
+    uint4* __restrict__ dst = (uint4*)(scratch + (int64_t)t * g.slot_elems);
+    for (int w4 = threadIdx.x; 4 * w4 < g.win_words_padded; w4 += GRID_THREADS)
+      dst[w4] = ((const uint4*)win)[w4];
+    return;
+  }
   // store the window as 27 regions
   uint16_t* __restrict__ img = scratch + (int64_t)t * g.slot_elems;
   const int sz3[3] = {sx, sy, sz};
@@ -519,8 +594,8 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
                                   : lz + 1 < ez ? plane_step
                                   : e + 1 < n_r ? type_step
                                                 : 0u);
-      const uint32_t v0 = (win[wc0 >> 1] >> ((wc0 & 1u) << 4)) & 0xFFFFu;
-      const uint32_t v1 = (win[wc1 >> 1] >> ((wc1 & 1u) << 4)) & 0xFFFFu;
+      const uint32_t v0 = window_half<PAIR>(win, wc0, wcl);
+      const uint32_t v1 = window_half<PAIR>(win, wc1, wcl);
       dst[j] = v0 | (v1 << 16);
     }
   }
@@ -607,6 +682,86 @@ __global__ void __launch_bounds__(SUM_THREADS) k_tiles_sum(const KParams* __rest
         sum = 0xFFFFu;
       }
       counts16[cell] = (uint16_t)sum;
+    }
+  }
+}
+
+// Two monomer types: the images are the windows themselves (word c holds
+// both types' counters of window cell c).  One lane per cell: the tile's own
+// window and, on each axis where the cell lies in a band of width h, the
+// neighbouring window whose halo covers it; both types summed at once.
+template <int BITS>
+__global__ void __launch_bounds__(SUM_THREADS) k_tiles_sum_pair(const KParams* __restrict__ Ps,
+                                                                int sys, TileGeo g,
+                                                                const uint32_t* __restrict__ img) {
+  KPC& kp = param(Ps, (uint32_t)sys);
+  SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
+  SCMF_GLOBAL uint16_t* __restrict__ counts16 = gptr((uint16_t*)kp.counts);
+  uint32_t* __restrict__ overflow = (uint32_t*)kp.overflow;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int v0 = t % g.nt[0], v1 = (t / g.nt[0]) % g.nt[1], v2 = t / (g.nt[0] * g.nt[1]);
+  const int s0 = tile_len(g.q[0], g.rem[0], v0), s1 = tile_len(g.q[1], g.rem[1], v1),
+            s2 = tile_len(g.q[2], g.rem[2], v2);
+  const int f0 = tile_first(g.q[0], g.rem[0], v0), f1 = tile_first(g.q[1], g.rem[1], v1),
+            f2 = tile_first(g.q[2], g.rem[2], v2);
+  // neighbouring tiles along each axis (periodic) and the length of the one below
+  const int d0 = v0 == 0 ? g.nt[0] - 1 : v0 - 1, u0 = v0 == g.nt[0] - 1 ? 0 : v0 + 1;
+  const int d1 = v1 == 0 ? g.nt[1] - 1 : v1 - 1, u1 = v1 == g.nt[1] - 1 ? 0 : v1 + 1;
+  const int d2 = v2 == 0 ? g.nt[2] - 1 : v2 - 1, u2 = v2 == g.nt[2] - 1 ? 0 : v2 + 1;
+  const int sd0 = tile_len(g.q[0], g.rem[0], d0), sd1 = tile_len(g.q[1], g.rem[1], d1),
+            sd2 = tile_len(g.q[2], g.rem[2], d2);
+  const int64_t words = g.slot_elems / 2;  // 32-bit words of one image
+  const int64_t zero_word = (int64_t)g.ntiles * words;
+  const uint32_t W0 = (uint32_t)g.W[0], W01 = (uint32_t)(g.W[0] * g.W[1]);
+  const int ncell = s0 * s1 * s2;
+  const float isx = 1.0f / (float)s0, isy = 1.0f / (float)s1;
+  for (int e = threadIdx.x; e < ncell; e += SUM_THREADS) {
+    const int lyz = small_div(e, isx);
+    const int i0 = e - lyz * s0;
+    const int i2 = small_div(lyz, isy);
+    const int i1 = lyz - i2 * s1;
+    // per axis: the cell's coordinate in the own window, and (if the cell is
+    // in the low or high band) the neighbour and the coordinate in its window
+    const bool lo0 = i0 < g.h[0], hi0 = i0 >= s0 - g.h[0];
+    const bool lo1 = i1 < g.h[1], hi1 = i1 >= s1 - g.h[1];
+    const bool lo2 = i2 < g.h[2], hi2 = i2 >= s2 - g.h[2];
+    const int n0 = lo0 ? d0 : u0, c0n = lo0 ? sd0 + g.h[0] + i0 : i0 - (s0 - g.h[0]);
+    const int n1 = lo1 ? d1 : u1, c1n = lo1 ? sd1 + g.h[1] + i1 : i1 - (s1 - g.h[1]);
+    const int n2 = lo2 ? d2 : u2, c2n = lo2 ? sd2 + g.h[2] + i2 : i2 - (s2 - g.h[2]);
+    const bool has0 = lo0 || hi0, has1 = lo1 || hi1, has2 = lo2 || hi2;
+    const uint32_t cell = (uint32_t)((f0 + i0) + kp.nx * ((f1 + i1) + kp.ny * (f2 + i2)));
+    // all eight loads issued at once; an absent contribution reads the
+    // zeroed word after the last image
+    uint32_t wv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const bool a = c & 1, b = c & 2, z = c & 4;
+      const bool live = (!a || has0) && (!b || has1) && (!z || has2);
+      const int w0 = a ? n0 : v0, w1 = b ? n1 : v1, w2 = z ? n2 : v2;
+      const uint32_t x0 = (uint32_t)(a ? c0n : g.h[0] + i0);
+      const uint32_t x1 = (uint32_t)(b ? c1n : g.h[1] + i1);
+      const uint32_t x2 = (uint32_t)(z ? c2n : g.h[2] + i2);
+      const int64_t u = w0 + g.nt[0] * (w1 + g.nt[1] * w2);
+      wv[c] = img[live ? u * words + x0 + W0 * x1 + W01 * x2 : zero_word];
+    }
+    uint32_t sum0 = BITS == 32 ? counts[cell] : (uint32_t)counts16[cell];
+    uint32_t sum1 = BITS == 32 ? counts[kp.n_cells + cell] : (uint32_t)counts16[kp.n_cells + cell];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      sum0 += wv[c] & 0xFFFFu;
+      sum1 += wv[c] >> 16;
+    }
+    if (BITS == 32) {
+      counts[cell] = sum0;
+      counts[kp.n_cells + cell] = sum1;
+    } else {
+      if ((sum0 | sum1) > 0xFFFFu) {
+        atomicOr(overflow, 1u);
+        sum0 = min(sum0, 0xFFFFu);
+        sum1 = min(sum1, 0xFFFFu);
+      }
+      counts16[cell] = (uint16_t)sum0;
+      counts16[kp.n_cells + cell] = (uint16_t)sum1;
     }
   }
 }

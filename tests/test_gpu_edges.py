@@ -112,20 +112,23 @@ def test_tiled_gridding_general_cells(bits):
     run_pair(spec, sweeps=4, seed=14, density_bits=bits, sort_every=2)
 
 
-@pytest.mark.parametrize("bits", [16, 32])
-def test_crowded_cell(bits):
-    """70,000 single beads in one cell: the tile holds more than 65,535 beads,
-    so its window is normalised while binning.  uint32 counts stay exact;
-    uint16 counts raise the overflow flag."""
-    banner(f"crowded cell u{bits}")
+@pytest.mark.parametrize("bits,crowd", [(16, 0), (32, 0), (32, 1)])
+def test_crowded_cell(bits, crowd):
+    """70,000 single beads of one type in one cell: the tile holds more than
+    65,535 beads, so its window is normalised while binning (for two types
+    both counters of a cell share a word: type 1 is the high half).  uint32
+    counts stay exact; uint16 counts raise the overflow flag."""
+    banner(f"crowded cell u{bits} type {crowd}")
+    many, few = W.homopolymer(1, 70_000, crowd), W.homopolymer(1, 5_000, 1 - crowd)
     spec = SystemSpec(n_types=2, xn=W.xn_matrix(2, 40.0, 15.0), L=[2.0, 2.0, 2.0],
                       n=[16, 16, 16], dtNxi=0.1, nref=16,
-                      polymers=[W.homopolymer(1, 70_000, 0), W.homopolymer(1, 5_000, 1)])
+                      polymers=[many, few] if crowd == 0 else [few, many])
     g = scmf.System(spec, make_opts(seed=15, density_bits=bits, sort_every=1))
     try:
         o = orc.Oracle(spec, g.opts)
         pos = o.init_coords()
-        pos[:70_000] = np.array([0.51, 1.26, 0.77], np.float32)  # all type-0 beads in one cell
+        first = 0 if crowd == 0 else 5_000
+        pos[first:first + 70_000] = np.array([0.51, 1.26, 0.77], np.float32)  # one cell
         g.set_positions(pos)
         g.grid_density()
         want = o.grid(pos)
