@@ -361,6 +361,9 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
     spare_s = 0u;
   }
   const Axis ax0 = axis(kp, 0), ax1 = axis(kp, 1), ax2 = axis(kp, 2);
+  // (kept in registers: the loop's global atomics would make the compiler
+  // reload them from the parameter block)
+  const uint32_t n_cells = (uint32_t)kp.n_cells, nx = (uint32_t)kp.nx, ny = (uint32_t)kp.ny;
   uint32_t missed = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
   const bool safe = attempt == 1;        // the normalising pass
@@ -502,16 +505,20 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
           // (the returned words matter only when tracked)
           const uint32_t got4 = (got[0] | got[1]) | (got[2] | got[3]);
           seen |= track ? got4 : 0u;
-          if (!safe) {
+          // beads outside the window (one wave-uniform test per pass)
+          bool out[4];
+#pragma unroll
+          for (int e = 0; e < 4; ++e) out[e] = e >= elo && e < ehi && sp[e] >= OUT_OF_WINDOW;
+          if (!safe && __ballot(out[0] | out[1] | out[2] | out[3])) {
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-              if (e >= elo && e < ehi && sp[e] >= OUT_OF_WINDOW) {
+              if (out[e]) {
                 int xi, yi, zi;
                 decode(e, &xi, &yi, &zi);
                 ++missed;
                 const uint32_t type = PAIR ? ty[e] >> 4 : ty[e];
                 add_global<BITS>(counts, overflow,
-                                 type * kp.n_cells + (uint32_t)(xi + kp.nx * (yi + kp.ny * zi)), 1u);
+                                 type * n_cells + (uint32_t)(xi + nx * (yi + ny * zi)), 1u);
               }
             }
           }
