@@ -379,8 +379,8 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
                          kernel=("k_schedule+" if pre else "") + sweep_kernel),
         "passes": {"grid": dict(roof(grid_bytes, tm.grid_ms,
                                      traffic_of(grid_keys) if tiled else None),
-                                kernel=("k_zero_counts+k_grid_tiles(cells)+k_tiles_sum"
-                                        if from_cells else "k_zero_counts+k_grid_tiles+k_tiles_sum")
+                                kernel=("k_zero_counts+k_grid_tiles" + ("(cells)+" if from_cells else "+")
+                                        + ("k_tiles_sum_pair" if spec.n_types == 2 else "k_tiles_sum"))
                                 if tiled else "k_zero_counts+k_grid"),
                    "omega": dict(roof(omega_bytes, tm.omega_ms, traffic_of([omega_key])),
                                  kernel=omega_key)},
