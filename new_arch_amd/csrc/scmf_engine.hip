@@ -1344,16 +1344,14 @@ int launch_sweep(scmf_sys* s) {
 // fill the GPU.
 size_t tile_lds_cap() { return (size_t)160 << 10; }
 
-void choose_tiles(scmf_sys* s) {
-  const KParams& kp = s->kp;
-  s->tile_lds = 0;
-  std::vector<std::pair<int, int64_t>> lens;
+// The plan depends only on the grid, the cell spacing, the chain lengths
+// (with their bead counts), Nref and the number of monomer types, so the
+// device-memory estimate of a description uses it too.  Returns the LDS
+// bytes of k_grid_tiles and fills *out, or returns 0 if no tiling fits.
+size_t plan_tiles(const int n[3], const double d[3], std::vector<std::pair<int, int64_t>> lens,
+                  int nref, int T, TileGeo* out) {
   int64_t total = 0;
-  for (int p = 0; p < kp.P; ++p) {
-    const int64_t b = kp.bead_base[p + 1] - kp.bead_base[p];
-    lens.push_back({kp.len[p], b});
-    total += b;
-  }
+  for (auto& l : lens) total += l.second;
   std::sort(lens.begin(), lens.end());
   int nmed = lens.back().first;
   int64_t acc = 0;
@@ -1364,14 +1362,12 @@ void choose_tiles(scmf_sys* s) {
       break;
     }
   }
-  const double reach = 0.75 * sqrt((double)std::max(nmed - 1, 1) / (double)std::max(s->desc.nref - 1, 1));
-  const int n[3] = {kp.nx, kp.ny, kp.nz};
-  const double d[3] = {kp.dx, kp.dy, kp.dz};
+  const double reach = 0.75 * sqrt((double)std::max(nmed - 1, 1) / (double)std::max(nref - 1, 1));
   int hh[3];
   for (int a = 0; a < 3; ++a) hh[a] = std::max(1, std::min(16, (int)ceil(reach / d[a])));
   // LDS: the window, the per-axis offset tables, the type table, slack
   const size_t tables = (size_t)4 * (n[0] + n[1] + n[2]);
-  if (tables > ((size_t)16 << 10)) return;
+  if (tables > ((size_t)16 << 10)) return 0;
   const size_t lds_cap = std::min(((size_t)160 << 10) - MONO_LDS - 512 - tables, tile_lds_cap());
   // candidate tile counts per axis: one tile, or pieces no shorter than
   // twice the halo and no longer than 64 cells
@@ -1401,7 +1397,7 @@ void choose_tiles(scmf_sys* s) {
           wc *= g.W[a];
           ic *= g.E[a];
         }
-        const size_t bytes = (size_t)((((wc * s->T + 1) / 2) + 3) & ~(int64_t)3) * 4;
+        const size_t bytes = (size_t)((((wc * T + 1) / 2) + 3) & ~(int64_t)3) * 4;
         if (bytes > lds_cap) continue;
         const int64_t ntiles = (int64_t)n0 * n1 * n2;
         const int per_cu =
@@ -1415,25 +1411,34 @@ void choose_tiles(scmf_sys* s) {
           bg = g;
           bg.ntiles = (int)ntiles;
           bg.win_cells = (int)wc;
-          bg.win_words = (int)((wc * s->T + 1) / 2);
+          bg.win_words = (int)((wc * T + 1) / 2);
           bg.win_words_padded = (bg.win_words + 3) & ~3;
         }
       }
-  if (best <= 0) return;
+  if (best <= 0) return 0;
   // region offsets of a window image: each region sized for the largest tile
   int off = 0;
   for (int r = 0; r < 27; ++r) {
     const int rr[3] = {r % 3, (r / 3) % 3, r / 9};
-    int64_t e = s->T;
+    int64_t e = T;
     for (int a = 0; a < 3; ++a) e *= rr[a] == 1 ? bg.E[a] : bg.h[a];
     bg.reg_off[r] = off;
     off += (int)((e + 1) & ~(int64_t)1);
   }
   // (two monomer types store the window itself: win_words_padded words)
-  if (s->T == 2) off = std::max(off, 2 * bg.win_words_padded);
+  if (T == 2) off = std::max(off, 2 * bg.win_words_padded);
   bg.slot_elems = (off + 63) & ~63;
-  s->geo = bg;
-  s->tile_lds = (size_t)bg.win_words_padded * 4 + tables;
+  *out = bg;
+  return (size_t)bg.win_words_padded * 4 + tables;
+}
+
+void choose_tiles(scmf_sys* s) {
+  const KParams& kp = s->kp;
+  std::vector<std::pair<int, int64_t>> lens;
+  for (int p = 0; p < kp.P; ++p) lens.push_back({kp.len[p], kp.bead_base[p + 1] - kp.bead_base[p]});
+  const int n[3] = {kp.nx, kp.ny, kp.nz};
+  const double d[3] = {kp.dx, kp.dy, kp.dz};
+  s->tile_lds = plan_tiles(n, d, lens, s->desc.nref, s->T, &s->geo);
 }
 
 // The tiled gridding kernel: from the sweep's cells or from positions,
@@ -2574,8 +2579,19 @@ void fill_summary(const scmf_desc* d, scmf_summary* out) {
                    8ull * ((chains + BLOCK - 1) / BLOCK) + 4096;
   if (nmax <= 255 && chains >= 65536)  // schedule records
     fixed += (uint64_t)(2 * nmax + 3) * 8ull * (uint64_t)((chains + 7) / 8);
-  if (beads >= ((int64_t)1 << 23))  // automatic ordering: second position buffer, cells,
-    fixed += 16ull * B4 + 16ull * chains + 8ull * TC;  // keys and maps, window images
+  if (beads >= ((int64_t)1 << 23)) {  // automatic ordering: second position buffer, cells,
+    fixed += 16ull * B4 + 16ull * chains;  // keys and maps
+    // window images and tile counters of the tiled gridding, from the same plan
+    std::vector<std::pair<int, int64_t>> lens;
+    for (int p = 0; p < d->n_poly_types; ++p)
+      lens.push_back({(int)d->poly_length[p], d->n_polymers[p] * (int64_t)d->poly_length[p]});
+    const int n[3] = {d->n[0], d->n[1], d->n[2]};
+    const double dd[3] = {d->L[0] / d->n[0], d->L[1] / d->n[1], d->L[2] / d->n[2]};
+    TileGeo g{};
+    if (plan_tiles(n, dd, lens, d->reference_Nbeads, d->n_mono_types, &g))
+      fixed += 2ull * ((uint64_t)g.ntiles * (uint64_t)g.slot_elems + 64) +
+               12ull * (uint64_t)d->n_poly_types * (uint64_t)(g.ntiles + 1);
+  }
   fixed += 26ull << 21;  // up to 26 device allocations, each rounded to 2 MiB
   out->device_bytes_16 = fixed + 4ull * ((TC + 1) / 2);
   out->device_bytes_32 = fixed + 4ull * TC;
