@@ -204,10 +204,12 @@ def tiled_spec(grid, T=2, N=16):
 
 
 @pytest.mark.parametrize("bits", [16, 32])
-@pytest.mark.parametrize("grid,T", [((27, 31, 40), 2), ((32, 32, 32), 3), ((64, 24, 33), 2)])
+@pytest.mark.parametrize("grid,T", [((27, 31, 40), 2), ((32, 32, 32), 3), ((64, 24, 33), 2),
+                                    ((48, 40, 10), 2)])
 def test_tiled_gridding_bit_exact(grid, T, bits):
     """Spatially ordered chains, windows with partial tiles, outliers on the
-    global-atomic fallback; counts equal the oracle's at every stage."""
+    global-atomic fallback; counts equal the oracle's at every stage.  (A slab
+    of 10 cells is thinner than two halos: one tile and no halo along z.)"""
     banner(f"tiled grid {grid} T={T} u{bits}")
     spec = tiled_spec(grid, T)
     g, o = pair(spec, seed=5, density_bits=bits, sort_every=3)
