@@ -356,9 +356,11 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   const bool checked = pop > 65535;
   __shared__ uint32_t redo_s;
   __shared__ uint32_t spare_s;  // target of the atomics of beads not counted in the window
+  __shared__ uint32_t missed_s;  // beads of the tile counted with global atomics
   if (threadIdx.x == 0) {
     redo_s = 0u;
     spare_s = 0u;
+    missed_s = 0u;
   }
   const Axis ax0 = axis(kp, 0), ax1 = axis(kp, 1), ax2 = axis(kp, 2);
   // (kept in registers: the loop's global atomics would make the compiler
@@ -547,10 +549,12 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
     if ((threadIdx.x & 63) == 0 && (seen & 0x80008000u)) atomicOr(&redo_s, 1u);
   }
   }
-  // beads counted globally: one atomic per wave
+  // beads counted globally: summed in LDS, one global atomic per tile (one
+  // per wave on the single counter contended when tiles end together)
   for (int off = 32; off > 0; off >>= 1) missed += __shfl_xor(missed, off);
-  if ((threadIdx.x & 63) == 0 && missed) atomicAdd(fallback, (unsigned long long)missed);
+  if ((threadIdx.x & 63) == 0 && missed) atomicAdd(&missed_s, missed);
   __syncthreads();
+  if (threadIdx.x == 0 && missed_s) atomicAdd(fallback, (unsigned long long)missed_s);
   if (PAIR) {
     // the window verbatim, 16 bytes a lane (images are 128-byte aligned)
     uint4* __restrict__ dst = (uint4*)(scratch + (int64_t)t * g.slot_elems);
