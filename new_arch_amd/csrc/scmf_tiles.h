@@ -8,11 +8,12 @@
 //      slot index and the per-bead monomer types never move);
 //   2. one workgroup per tile bins its chains' beads into an LDS window
 //      (the tile plus a halo of h cells per side, uint16 counters packed in
-//      pairs), four beads per lane from 16-byte loads, and stores the window
-//      as 27 contiguous regions (low halo, interior, high halo per axis);
-//   3. one workgroup per tile sums, for each of its cells, the regions of
-//      the (up to 27) windows that cover it — every region is read once,
-//      contiguously.
+//      pairs; with two monomer types the pair is a cell's two types), four
+//      beads per lane from 16-byte loads, and stores the window: verbatim
+//      for two types, else as 27 contiguous regions (low halo, interior,
+//      high halo per axis);
+//   3. one workgroup per tile sums, for each of its cells, the windows that
+//      cover it (its own and up to seven neighbours).
 // A bead outside its tile's window (a chain that drifted since the last
 // ordering) is added with a global atomic instead, so results never depend
 // on how well ordered the chains are; only speed does.
@@ -273,10 +274,11 @@ __device__ void normalise_window(uint32_t* win, const TileGeo& g, KPC& kp, int o
 }
 
 // One workgroup per tile: bin the beads of the tile's chains into the LDS
-// window, then store the window as 27 regions.  Beads outside the window go
-// straight to the final counts with a global atomic.  Positions are read
-// four beads per lane (16-byte loads; the arrays are padded to a multiple of
-// four beads), the next four in flight while the current ones are binned.
+// window, then store the window (verbatim for two monomer types, else as 27
+// regions).  Beads outside the window go straight to the final counts with a
+// global atomic.  Positions are read four beads per lane (16-byte loads; the
+// arrays are padded to a multiple of four beads), later passes' loads in
+// flight while the current ones are binned.
 // With CELLS the cells the last sweep stored for the same positions are read
 // instead (4 bytes a bead, no binning).
 //
@@ -397,9 +399,9 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
     auto bin_range = [&](auto tab_c) {
       constexpr bool TAB = decltype(tab_c)::value;
       // Each lane takes four beads per pass and keeps the loads of the next
-      // PF passes in flight while it bins these (one pass in flight per lane
-      // would cap the whole GPU near 2 TB/s at HBM latency).  Loads are
-      // issued unconditionally (lanes past the range re-read the last group
+      // PF passes in flight while it bins these (one pass a lane is ~4 MB in
+      // flight on the whole GPU, ~16 KB per CU).  Loads are issued
+      // unconditionally (lanes past the range re-read the last group
       // of the range), so the wait before binning counts only the current
       // group and the later ones stay in flight.
       constexpr int PF = CELLS ? 4 : 2;
