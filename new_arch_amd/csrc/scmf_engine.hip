@@ -1350,6 +1350,7 @@ size_t tile_lds_cap() { return (size_t)160 << 10; }
 // bytes of k_grid_tiles and fills *out, or returns 0 if no tiling fits.
 size_t plan_tiles(const int n[3], const double d[3], std::vector<std::pair<int, int64_t>> lens,
                   int nref, int T, TileGeo* out) {
+  if (lens.empty()) return 0;
   int64_t total = 0;
   for (auto& l : lens) total += l.second;
   std::sort(lens.begin(), lens.end());
