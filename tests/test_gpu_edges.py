@@ -1,8 +1,9 @@
 """Edge cases of the HIP engine against the oracle, bit for bit: chains longer
 than 255 beads (16-bit schedule indices), one-, two- and three-bead chains,
 architectures with no chains, one and eight monomer types, tiled gridding
-with cells that are not a power of two, and a cell holding more beads than a
-16-bit counter (the window normalisation of the tiled gridding)."""
+with cells that are not a power of two or architectures longer than its type
+table, and a cell holding more beads than a 16-bit counter (the window
+normalisation of the tiled gridding)."""
 import numpy as np
 import pytest
 
@@ -139,6 +140,48 @@ def test_crowded_cell(bits, crowd):
         else:
             assert g.stats().overflow == 1
         assert g.stats().sorts >= 1
+    finally:
+        g.close()
+
+
+@pytest.mark.parametrize("bits", [16, 32])
+def test_tiled_gridding_architecture_longer_than_type_table(bits):
+    """An architecture longer than the tiled gridding's LDS type table (4096
+    monomers) reads its types from global memory, a short one beside it from
+    the table; both in the same tiles, two monomer types (paired counters)."""
+    banner(f"tiled, N=4500 beside N=32, u{bits}")
+    N = 4500
+    spec = SystemSpec(n_types=2, xn=W.xn_matrix(2, 40.0, 15.0), L=[3.0, 3.0, 3.0],
+                      n=[24, 24, 24], dtNxi=0.1, nref=N,
+                      polymers=[PolymerType(length=N, count=24, monotype=[(i // 9) % 2 for i in range(N)]),
+                                W.diblock(32, 0.5, 2000)])
+    # (the initial walk stops at 4096 beads a chain: positions from a random walk here)
+    rng = np.random.default_rng(16)
+    chains = [(N, 24), (32, 2000)]
+    pos = []
+    for n, count in chains:
+        start = rng.uniform(0.0, 3.0, (count, 1, 3))
+        steps = rng.normal(0.0, (3.0 * (N - 1)) ** -0.5, (count, n - 1, 3))
+        walk = np.concatenate([start, start + np.cumsum(steps, axis=1)], axis=1)
+        pos.append(walk.reshape(-1, 3))
+    pos = np.ascontiguousarray(np.concatenate(pos).astype(np.float32))
+    g = scmf.System(spec, make_opts(seed=16, density_bits=bits, sort_every=1, init_coords=False))
+    try:
+        o = orc.Oracle(spec, g.opts)
+        g.set_positions(pos)
+        g.grid_density()
+        c = g.density()
+        assert np.array_equal(c, o.grid(pos))
+        assert g.stats().sorts >= 1                 # the tiled path
+        g.update_omega()
+        om = g.omega()
+        acc = o.sweep(pos, om, 0)
+        g.reset_stats()
+        g.mc_sweep(1)
+        assert g.stats().accepts == acc
+        assert same_bits(g.positions(), pos)
+        g.grid_density()
+        assert np.array_equal(g.density(), o.grid(pos))
     finally:
         g.close()
 
