@@ -205,7 +205,7 @@ def tiled_spec(grid, T=2, N=16):
 
 @pytest.mark.parametrize("bits", [16, 32])
 @pytest.mark.parametrize("grid,T", [((27, 31, 40), 2), ((32, 32, 32), 3), ((64, 24, 33), 2),
-                                    ((48, 40, 10), 2)])
+                                    ((48, 40, 10), 2), ((40, 36, 30), 1)])
 def test_tiled_gridding_bit_exact(grid, T, bits):
     """Spatially ordered chains, windows with partial tiles, outliers on the
     global-atomic fallback; counts equal the oracle's at every stage.  (A slab
