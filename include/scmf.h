@@ -108,7 +108,8 @@ typedef struct scmf_summary {
   int64_t n_chains, n_beads, n_cells;
   double xn[64];             /* first min(T*T, 64) entries of xn[t + T*s] */
   uint64_t device_bytes_16;  /* device memory a system needs with uint16 counts
-                                (ordering buffers included from 2^23 beads, where it
+                                (ordering buffers, the tiled gridding's window images
+                                among them, included from 2^23 beads, where ordering
                                 is automatic; not the optional analysis and output
                                 buffers) */
   uint64_t device_bytes_32;  /* ... with uint32 counts */
