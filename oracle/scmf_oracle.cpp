@@ -13,10 +13,24 @@
 // streams of new_arch_amd/csrc/scmf_prims.h; those primitives are pinned
 // separately (Philox known-answer vectors, libm error bounds).
 //
-// Parity pins: binning (brute force + invariants), omega (the reference's own
-// density_*.dat files decode to integer counts through orc_omega), move
-// (hand-computed single-move cases, ideal-chain closed form, detailed balance
-// on a dimer), init walk (closed-form bond variance).  See tests/.
+// Parity pins (tests/test_oracle_pins.py, tests/test_prims.py):
+//   binning      exact rational arithmetic, hand-derived edge cases, periodicity,
+//                invariants (sum of counts, per-type sums, order independence);
+//   omega        the reference's own density_*.dat files: their omega columns are
+//                reproduced from the integer counts they decode to; closed forms;
+//   derived      harm = 3 (Nref - 1) / 2, A = dtNxi / Nref, R = sqrt(2 A) with
+//                N != Nref;
+//   move         single moves of a 3-bead chain with N != Nref computed by hand
+//                from the bond energy, the force-biased Gaussian proposal and the
+//                Metropolis-Hastings ratio (smart and trial moves, the uniform just
+//                above and just below exp(-dE), the drift and the noise separately,
+//                the fp64 new cell across a cell edge, the bead pick); relaxation
+//                to the ideal chain with harm from Nref (N = Nref and N != Nref);
+//                detailed balance of dimers (chi-square of the bond length);
+//                Boltzmann occupancy of a two-level field;
+//   init walk    bond variance 1/(3 (N - 1)) from the chain length with N != Nref,
+//                <Re^2> = 1;
+//   streams      Philox known-answer vectors, special functions against libm.
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
