@@ -1,7 +1,10 @@
 """Exact checkpoint/restart through the C ABI: a run restored from a
 checkpoint continues bit for bit (positions, counts, omega, stream position),
-including the reference omega cadence that depends on the step number; a
-checkpoint of another system or a damaged file is refused."""
+including the reference omega cadence that depends on the step number, and
+ends where the oracle's uninterrupted run of the same steps ends (the
+reference's restart path is init_config.cu:930-987 with io.cpp:95-118, which
+loses digits to %g; this one does not); a checkpoint of another system or a
+damaged file is refused."""
 import os
 
 import numpy as np
@@ -10,6 +13,7 @@ import pytest
 from new_arch_amd import scmf
 from new_arch_amd import workloads as W
 from new_arch_amd.abi import make_opts
+from oracle import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
@@ -44,6 +48,22 @@ def test_restored_run_continues_bit_for_bit(tmp_path, bits, sort_every):
     assert np.array_equal(a.density(), b.density())
     assert same_bits(a.omega(), b.omega())
     assert a.stats().sweeps == b.stats().sweeps
+    # and both equal the oracle's uninterrupted run of the same 14 steps
+    # (test.cpp:37-43 with the reference cadence: no omega update at step 10)
+    o = orc.Oracle(spec, make_opts(seed=21, **opts))
+    pos = o.init_coords()
+    counts = o.grid(pos)
+    om = o.omega(counts).astype(np.float32)
+    acc = 0
+    for step in range(1, 15):
+        acc += o.sweep(pos, om, step - 1)
+        counts = o.grid(pos)
+        if step % 10:
+            om = o.omega(counts).astype(np.float32)
+    assert same_bits(b.positions(), pos)
+    assert np.array_equal(b.density(), counts)
+    assert same_bits(b.omega(), om)
+    assert a.stats().accepts == acc
     a.close()
     b.close()
 

@@ -342,6 +342,39 @@ def test_largest_blend_full_size():
     full_size_check(spec, 32, [na, nb])
 
 
+@pytest.mark.parametrize("sorted_", [False, True])
+def test_re2_matches_oracle(sorted_):
+    """<Re^2> over chains of three architectures (first to last bead, test.cpp:45-67
+    generalised) after a few sweeps, from chains in reference order and from
+    spatially reordered slots: equal to the oracle's up to fp64 summation order."""
+    banner(f"re2 sorted={sorted_}")
+    spec = W.mixed_arch(scale=8)
+    g, o = pair(spec, seed=404, sort_every=2 if sorted_ else -1)
+    pos = o.init_coords()
+    g.grid_density()
+    g.update_omega()
+    om = g.omega()
+    for s in range(4):
+        g.mc_sweep(1)
+        o.sweep(pos, om, s)
+        if sorted_:
+            g.sort_chains()
+    assert same_bits(g.positions(), pos)
+    assert (g.stats().sorts > 0) == sorted_
+    want = o.re2(pos)
+    got = g.stats().re2
+    assert abs(got - want) <= 1e-12 * want, (got, want)
+    # a chain's end-to-end vector uses its own first and last bead: moving the
+    # last bead of one star (13 beads) changes the mean by exactly its share
+    p2 = pos.copy()
+    base = spec.polymers[0].count * 24 + spec.polymers[1].count * 20
+    p2[base + 12] += np.float32(0.5)
+    g.set_positions(p2)
+    assert abs(g.stats().re2 - o.re2(p2)) <= 1e-12 * want
+    assert g.stats().re2 != got
+    g.close()
+
+
 def test_copy_errors():
     banner("copy errors")
     g = scmf.System(W.tiny(), make_opts(seed=1))
