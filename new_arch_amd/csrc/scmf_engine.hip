@@ -187,14 +187,13 @@ __device__ __forceinline__ void bond_force(const N& nb, double harm2, float px, 
 // One move of mc_polymer_move_arr (device_function.cu:747-790) with
 // trial_move_smc_arr (:637-717), calc_delta_energy_arr (:562-588),
 // calc_delta_bonded_energy_arr (:509-561) and som_accept (:444-454) inlined.
-// Returns 1 if accepted.
+// Move i of the sweep on bead ibead (its pick, drawn by the caller from the
+// move's SLOT_PICK block).  Returns 1 if accepted.
 template <bool LINEAR, class KP, class V>
 __device__ __forceinline__ uint32_t one_move_n(const KP& kp, const V& c, int p, int length,
                                                uint32_t chain, uint32_t sweep, uint32_t i,
+                                               int ibead,
                                                const SCMF_GLOBAL float* __restrict__ omega) {
-  const u32x4 pick = stream_block(kp.seed, kp.system_id, chain, sweep, i, SLOT_PICK);
-  int ibead = (int)(u01_53(pick.v[0], pick.v[1]) * length);
-  if (ibead == length) ibead = length - 1;
   const uint32_t iwtype = (uint32_t)gptr((const int*)kp.monotype)[kp.mono_off[p] + ibead];
   const u32x4 disp = stream_block(kp.seed, kp.system_id, chain, sweep, i, SLOT_DISP);
   const float ox = c.X(ibead), oy = c.Y(ibead), oz = c.Z(ibead);
@@ -255,7 +254,7 @@ __device__ __forceinline__ uint32_t one_move_n(const KP& kp, const V& c, int p, 
   double delta_energy = energy;
   delta_energy += smc_deltaE;
   const double p_acc = exp_cw(-1.0 * delta_energy);
-  if ((p_acc > 1) || (p_acc > (double)u01_24f(pick.v[2]))) {
+  if ((p_acc > 1) || (p_acc > (double)accept_u01(disp))) {
     c.set(ibead, newx, newy, newz);
     return 1u;
   }
@@ -265,10 +264,25 @@ __device__ __forceinline__ uint32_t one_move_n(const KP& kp, const V& c, int p, 
 // The move with neighbour storage sized for the chain's architecture.
 template <class KP, class V>
 __device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, int p, int length,
-                                             uint32_t chain, uint32_t sweep, uint32_t i,
+                                             uint32_t chain, uint32_t sweep, uint32_t i, int ibead,
                                              const SCMF_GLOBAL float* __restrict__ omega) {
-  if (kp.linear[p]) return one_move_n<true>(kp, c, p, length, chain, sweep, i, omega);
-  return one_move_n<false>(kp, c, p, length, chain, sweep, i, omega);
+  if (kp.linear[p]) return one_move_n<true>(kp, c, p, length, chain, sweep, i, ibead, omega);
+  return one_move_n<false>(kp, c, p, length, chain, sweep, i, ibead, omega);
+}
+
+// The N moves of one chain in order, on one lane: one pick block per two moves.
+template <class KP, class V>
+__device__ __forceinline__ uint32_t chain_moves(const KP& kp, const V& c, int p, int length,
+                                                uint32_t chain, uint32_t sweep,
+                                                const SCMF_GLOBAL float* __restrict__ omega) {
+  uint32_t acc = 0;
+  u32x4 pk{};
+  for (int i = 0; i < length; i++) {
+    if ((i & 1) == 0) pk = stream_block(kp.seed, kp.system_id, chain, sweep, (uint32_t)i >> 1, SLOT_PICK);
+    const int ib = pick_bead(pick_u01(pk, (uint32_t)i), length);
+    acc += one_move(kp, c, p, length, chain, sweep, (uint32_t)i, ib, omega);
+  }
+  return acc;
 }
 
 __device__ __forceinline__ void wave_count(uint32_t acc, unsigned long long* accepts) {
@@ -306,7 +320,7 @@ __global__ void __launch_bounds__(BLOCK) k_sweep_global(const KParams* __restric
     const int64_t off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
     const GlobalChain c{x + off, y + off, z + off};
     const uint32_t id = gptr((const uint32_t*)kp.slot_chain)[chain];
-    for (int i = 0; i < length; i++) acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, omega);
+    acc = chain_moves(kp, c, p, length, id, sweep, omega);
   }
   wave_count(acc, (unsigned long long*)kp.accepts);
 }
@@ -342,7 +356,7 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
     }
     const LdsChain<64> c{sx, sy, sz};
     const uint32_t id = gptr((const uint32_t*)kp.slot_chain)[chain];
-    for (int i = 0; i < length; i++) acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, omega);
+    acc = chain_moves(kp, c, p, length, id, sweep, omega);
     for (int b = 0; b < length; ++b) {
       x[off + b] = sx[b * 64];
       y[off + b] = sy[b * 64];
@@ -371,11 +385,12 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
 // 61% -> 80%, by simulation of the picks.)
 //
 // LDS per chain (index type I: uint8 for N <= 255): positions 3N floats
-// (bead b at [b*G + slot]); lev[N] (pick, then round of each move);
-// W[N+2] (round per bead during the scan, bead b at W[b+1] with zero
-// sentinels around, then per-round fill counts); start[N+2] (moves per round
-// during the scan, then the first list entry of each round); list[N] (moves
-// by round).
+// (bead b at [b*G + slot]); lev[N] (the bead each move picks); W[N+2]
+// (round per bead during the scan, bead b at W[b+1] with zero sentinels
+// around, then per-round fill counts); start[N+2] (moves per round during
+// the scan, then the first list entry of each round); list[N] (moves by
+// round); rnd[N] (the round of each move).  With a schedule made by
+// k_schedule only start and list are kept.
 // (At most 96 VGPRs, i.e. five waves per SIMD: the LDS tiles allow about
 // as many, and the move is issue-bound, so occupancy is worth the spills.)
 // Byte k of the move-schedule record of a slot (records interleaved by
@@ -406,11 +421,12 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
   // only; otherwise the chain's lanes make it here from lev and W.
   const SCMF_GLOBAL uint8_t* __restrict__ sched = gptr((const uint8_t*)kp.sched);
   const bool pre = sched != nullptr;
-  const int per = pre ? 2 * nmax + 4 : 4 * nmax + 4;
+  const int per = pre ? 2 * nmax + 4 : 5 * nmax + 4;
   I* lev = book + slot * per;
   I* W = lev + nmax;
   I* start = pre ? lev : W + nmax + 2;
   I* list = start + nmax + 2;
+  I* rnd = list + nmax;  // the round of each move (in-sweep scan only)
   const int64_t chain = (int64_t)br.local * G + slot;
   const bool active = chain < kp.n_chains;
   int p = 0, length = 0;
@@ -438,11 +454,10 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
     }
     __syncthreads();
   } else {
-  for (int i = r; i < length; i += L) {
-    const u32x4 pick = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i, SLOT_PICK);
-    int ib = (int)(u01_53(pick.v[0], pick.v[1]) * length);
-    if (ib == length) ib = length - 1;
-    lev[i] = (I)ib;
+  for (int i = 2 * r; i < length; i += 2 * L) {  // one pick block per two moves
+    const u32x4 pick = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i >> 1, SLOT_PICK);
+    lev[i] = (I)pick_bead(pick_u01(pick, (uint32_t)i), length);
+    if (i + 1 < length) lev[i + 1] = (I)pick_bead(pick_u01(pick, (uint32_t)i + 1), length);
   }
   for (int k = r; k <= length + 1; k += L) W[k] = 0;
   for (int k = r; k <= length + 1; k += L) start[k] = 0;
@@ -458,7 +473,7 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
         int l = max(max((int)Wb[b - 1], (int)Wb[b]), (int)Wb[b + 1]) + 1;
         while ((int)start[l + 1] >= L) ++l;  // round full: the next one
         Wb[b] = (I)l;
-        lev[i] = (I)l;
+        rnd[i] = (I)l;
         start[l + 1] = (I)(start[l + 1] + 1);
         maxl = max(maxl, l);
       }
@@ -470,7 +485,7 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
         l += 1;
         while ((int)start[l + 1] >= L) ++l;
         Wb[b] = (I)l;
-        lev[i] = (I)l;
+        rnd[i] = (I)l;
         start[l + 1] = (I)(start[l + 1] + 1);
         maxl = max(maxl, l);
       }
@@ -480,7 +495,7 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
       W[l] = 0;
     }
     for (int i = 0; i < length; ++i) {
-      const int l = lev[i];
+      const int l = rnd[i];
       const int k = W[l];
       W[l] = (I)(k + 1);
       list[start[l] + k] = (I)i;
@@ -497,7 +512,20 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
     if (l <= maxl) {
       const int s0 = start[l], cl = (int)start[l + 1] - s0;
       const int k = j * L + r;
-      if (k < cl) acc += one_move(kp, c, p, length, id, sweep, (uint32_t)list[s0 + k], omega);
+      if (k < cl) {
+        const int i = list[s0 + k];
+        // the pick block's arithmetic at the head of the move measured faster
+        // than loading the bead from a longer schedule record (7.59 against
+        // 6.78 ms a sweep at 268M beads): it overlaps the latency of the loads
+        int ib;
+        if (pre) {
+          const u32x4 pb = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i >> 1, SLOT_PICK);
+          ib = pick_bead(pick_u01(pb, (uint32_t)i), length);
+        } else {
+          ib = (int)lev[i];
+        }
+        acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, ib, omega);
+      }
       if (++j * L >= cl) {
         ++l;
         j = 0;
@@ -533,7 +561,7 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
 template <int L, typename I>
 size_t sched_lds_bytes(int nmax) {
   constexpr int G = 64 / L;
-  return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (4 * nmax + 4) * sizeof(I);
+  return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (5 * nmax + 4) * sizeof(I);
 }
 
 // The same kernel reading a schedule made by k_schedule.
@@ -568,16 +596,16 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restr
   const int p = arch_of_chain(kp, chain);
   const int length = kp.len[p];
   const uint32_t id = gptr((const uint32_t*)kp.slot_chain)[chain];
-  uint8_t* const lev = sh + t;                           // [nmax]
-  uint8_t* const W = lev + nmax * SCHED_LANES;           // [nmax + 2], W[b + 1] = level of bead b
+  uint8_t* const lev = sh + t;                           // [nmax] round of each move
+  uint8_t* const W = lev + nmax * SCHED_LANES;           // [nmax + 2], W[b + 1] = round of bead b
   uint8_t* const start = W + (nmax + 2) * SCHED_LANES;   // [nmax + 2]
   auto at = [](uint8_t* a, int k) -> uint8_t& { return a[k * SCHED_LANES]; };
   SCMF_GLOBAL uint8_t* __restrict__ rec = gptr(kp.sched);
   const int stride = kp.sched_stride;
-  auto pick = [&](int i) {
-    const u32x4 w = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i, SLOT_PICK);
-    const int ib = (int)(u01_53(w.v[0], w.v[1]) * length);
-    return ib == length ? length - 1 : ib;
+  u32x4 blk{};
+  auto pick = [&](int i) {  // one pick block per two moves
+    if ((i & 1) == 0) blk = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i >> 1, SLOT_PICK);
+    return pick_bead(pick_u01(blk, (uint32_t)i), length);
   };
   for (int k = 0; k <= length + 1; ++k) {
     at(W, k) = 0;
@@ -616,7 +644,8 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restr
     const int l = at(lev, i);
     const int k = at(W, l);
     at(W, l) = (uint8_t)(k + 1);
-    rec[sched_index(chain, stride, length + 3 + at(start, l) + k)] = (uint8_t)i;
+    const int pos = at(start, l) + k;
+    rec[sched_index(chain, stride, length + 3 + pos)] = (uint8_t)i;
   }
 }
 
@@ -635,7 +664,12 @@ __global__ void __launch_bounds__(BLOCK) k_zero(uint32_t* __restrict__ p, int64_
   for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) p[i] = 0u;
 }
 
-// coord_to_density, device_function.cu:479-508, one lane per bead.
+// coord_to_density, device_function.cu:479-508, one lane per bead.  A wave's
+// beads are consecutive beads of one or two chains, so many share a count
+// word: the lanes add to each distinct word of the wave with one atomic (the
+// word's total, both uint16 halves at once), so the memory side sees one
+// atomic per distinct word instead of one per bead.  A uint16 half overflows
+// when its old value plus the wave's increment passes 0xFFFF.
 __global__ void __launch_bounds__(BLOCK) k_grid(const KParams* __restrict__ Ps,
                                                 const BlockRef* __restrict__ bt) {
   const BlockRef br = block_ref(bt);
@@ -645,16 +679,32 @@ __global__ void __launch_bounds__(BLOCK) k_grid(const KParams* __restrict__ Ps,
   const SCMF_GLOBAL float* __restrict__ z = gptr((const float*)kp.z);
   SCMF_GLOBAL uint32_t* __restrict__ words = gptr((uint32_t*)kp.counts);
   const bool b32 = kp.density_bits == 32;
+  const int lane = (int)(threadIdx.x & 63u);
   const int64_t stride = (int64_t)br.nblk * BLOCK;
   for (int64_t g = (int64_t)br.local * BLOCK + threadIdx.x; g < kp.n_beads; g += stride) {
     const uint32_t t = gptr((const uint8_t*)kp.btype)[g];
     const uint32_t cell = bin_cell(kp, x[g], y[g], z[g], t);
-    if (b32) {
-      atomicAdd((uint32_t*)&words[cell], 1u);
-    } else {
-      const uint32_t sh = (cell & 1u) * 16u;
-      const uint32_t old = atomicAdd((uint32_t*)&words[cell >> 1], 1u << sh);
-      if (((old >> sh) & 0xFFFFu) == 0xFFFFu) atomicOr((uint32_t*)kp.overflow, 1u);
+    const uint32_t word = b32 ? cell : cell >> 1;
+    const bool high = !b32 && (cell & 1u);
+    // the loop runs once per distinct word among the active lanes; `todo`
+    // is the same on every active lane, so they all take every ballot
+    unsigned long long todo = __ballot(1);
+    while (todo) {
+      const int leader = __ffsll(todo) - 1;
+      const uint32_t w = __builtin_amdgcn_readlane(word, leader);
+      const unsigned long long same = __ballot(word == w) & todo;
+      const uint32_t n_hi = (uint32_t)__popcll(__ballot(high) & same);
+      const uint32_t n_lo = (uint32_t)__popcll(same) - n_hi;
+      if (lane == leader) {
+        if (b32) {
+          atomicAdd((uint32_t*)&words[w], n_lo);
+        } else {
+          const uint32_t old = atomicAdd((uint32_t*)&words[w], n_lo | (n_hi << 16));
+          if ((old & 0xFFFFu) + n_lo > 0xFFFFu || (old >> 16) + n_hi > 0xFFFFu)
+            atomicOr((uint32_t*)kp.overflow, 1u);
+        }
+      }
+      todo &= ~same;
     }
   }
 }

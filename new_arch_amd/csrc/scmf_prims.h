@@ -69,10 +69,19 @@ SCMF_HD u32x4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, 
 }
 
 // Stream slots.  Counter layout: {chain, sweep, (index << 2) | slot, system}.
+//   SLOT_PICK, index k: the bead picks of moves 2k (words 0, 1) and 2k + 1
+//                       (words 2, 3), 53-bit doubles;
+//   SLOT_DISP, index i: move i's displacement from the top 24 bits of its words
+//                       (three normals for smart MC, three uniforms for trial
+//                       moves) and its accept uniform from the low bytes of
+//                       words 0, 1, 2 (bits no displacement uses);
+//   SLOT_INIT:          the initial configuration (start point, then one block
+//                       per placed bead).
+// A move thus costs one Philox block plus half of one for its pick.
 enum : uint32_t {
-  SLOT_PICK = 0,    // bead pick (53-bit double) + accept uniform (24-bit float)
-  SLOT_DISP = 1,    // displacement: three normals (smart MC) or three uniforms (trial)
-  SLOT_INIT = 2,    // initial configuration (start point, then one block per placed bead)
+  SLOT_PICK = 0,
+  SLOT_DISP = 1,
+  SLOT_INIT = 2,
   INIT_SWEEP = 0xFFFFFFFFu  // sweep word reserved for the initial configuration
 };
 
@@ -95,6 +104,23 @@ SCMF_HD float u01_24f_closed0(uint32_t a) { return (float)(a >> 8) * 5.960464477
 // (0, 1) double from 32 bits: (a + 1/2) 2^-32.  Exact.
 SCMF_HD double u01_32_open(uint32_t a) {
   return ((double)a + 0.5) * 2.3283064365386962890625e-10;
+}
+
+// The pick uniform of move i from its SLOT_PICK block (index i >> 1).
+SCMF_HD double pick_u01(const u32x4& w, uint32_t i) {
+  return (i & 1u) ? u01_53(w.v[2], w.v[3]) : u01_53(w.v[0], w.v[1]);
+}
+// The bead a move with pick uniform u picks on a chain of n beads:
+// (int)(u n), the last bead for u = 1 (device_function.cu:749-751).
+SCMF_HD int pick_bead(double u, int n) {
+  const int b = (int)(u * n);
+  return b == n ? n - 1 : b;
+}
+// The accept uniform of a move, (0, 1] with 24 bits, from the low bytes of
+// words 0, 1, 2 of its SLOT_DISP block.  Exact.
+SCMF_HD float accept_u01(const u32x4& disp) {
+  const uint32_t b = (disp.v[0] & 0xFFu) | ((disp.v[1] & 0xFFu) << 8) | ((disp.v[2] & 0xFFu) << 16);
+  return (float)(b + 1u) * 5.9604644775390625e-8f;
 }
 
 // ---------------------------------------------------------------- log

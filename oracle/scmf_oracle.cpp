@@ -210,23 +210,28 @@ inline double bonded_delta(const Sys& s, int p, int mono, int64_t poly_index, co
 // som_accept (:444-454) inlined.  Returns the number of accepted moves.
 // The random numbers of move i of a chain: the Philox streams the engine
 // uses, or arrays injected by the caller (any other generator).
+// Move i reads its pick from the SLOT_PICK block of index i / 2 (words 0, 1
+// for even i, 2, 3 for odd i) and its displacement and accept uniform from
+// the SLOT_DISP block of index i (scmf_prims.h).
 struct PhiloxDraws {
   const scmf_opts& o;
   uint32_t sweep;
   int64_t chain;
   u32x4 pick, disp;
+  uint32_t move = 0;
   void at(int i) {
-    pick = stream_block(o.seed, o.system_id, (uint32_t)chain, sweep, (uint32_t)i, SLOT_PICK);
+    move = (uint32_t)i;
+    pick = stream_block(o.seed, o.system_id, (uint32_t)chain, sweep, (uint32_t)i >> 1, SLOT_PICK);
     disp = stream_block(o.seed, o.system_id, (uint32_t)chain, sweep, (uint32_t)i, SLOT_DISP);
   }
-  double pick_u() const { return u01_53(pick.v[0], pick.v[1]); }
+  double pick_u() const { return pick_u01(pick, move); }
   void normals(float* rx, float* ry, float* rz) const { normal3f(disp, rx, ry, rz); }
   void trial(float* ux, float* uy, float* uz) const {
     *ux = u01_24f_closed0(disp.v[0]);
     *uy = u01_24f_closed0(disp.v[1]);
     *uz = u01_24f_closed0(disp.v[2]);
   }
-  float accept_u() const { return u01_24f(pick.v[2]); }
+  float accept_u() const { return accept_u01(disp); }
 };
 
 // Injected draws, indexed by move: move i of the chain whose first bead is

@@ -144,6 +144,42 @@ def test_crowded_cell(bits, crowd):
         g.close()
 
 
+@pytest.mark.parametrize("half", [0, 1])
+@pytest.mark.parametrize("n", [65_535, 65_536])
+def test_atomic_gridding_uint16_boundary(n, half):
+    """The atomic gridding (one atomic per distinct count word of a wave, both
+    uint16 halves of the word at once): n type-0 beads in one cell and 3,000 in
+    the cell beside it along x, whose count is the other half of the same
+    32-bit word (cells x = 4 and 5: the low and the high half), beside 2,000
+    type-1 beads spread out.  65,535 in one half is exact with no overflow;
+    65,536 raises the flag, in either half."""
+    banner(f"atomic gridding boundary n={n} half {half}")
+    spec = SystemSpec(n_types=2, xn=W.xn_matrix(2, 40.0, 15.0), L=[2.0, 2.0, 2.0],
+                      n=[16, 16, 16], dtNxi=0.1, nref=16,
+                      polymers=[W.homopolymer(1, n + 3_000, 0), W.homopolymer(1, 2_000, 1)])
+    g = scmf.System(spec, make_opts(seed=16, density_bits=16, sort_every=-1))
+    try:
+        o = orc.Oracle(spec, g.opts)
+        pos = o.init_coords()
+        low = np.array([0.51, 1.26, 0.77], np.float32)    # x cell 4
+        high = np.array([0.70, 1.26, 0.77], np.float32)   # x cell 5, same word
+        pos[:n] = high if half else low
+        pos[n:n + 3_000] = low if half else high
+        g.set_positions(pos)
+        g.grid_density()
+        want = o.grid(pos)
+        c = 4 + 10 * 16 + 6 * 256
+        assert want[c + half] == n and want[c + 1 - half] == 3_000
+        if n <= 65_535:
+            assert g.stats().overflow == 0
+            np.testing.assert_array_equal(g.density(), want)
+        else:
+            assert g.stats().overflow == 1
+        assert g.stats().sorts == 0
+    finally:
+        g.close()
+
+
 @pytest.mark.parametrize("bits", [16, 32])
 def test_tiled_gridding_architecture_longer_than_type_table(bits):
     """An architecture longer than the tiled gridding's LDS type table (4096

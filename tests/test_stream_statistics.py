@@ -36,7 +36,10 @@ def test_injected_sweep_matches_streams_when_fed_the_same_numbers():
     pos = o.init_coords()
     om = o.omega(o.grid(pos)).astype(np.float32)
     n = o.n_beads
-    # Rebuild the draws of sweep 0 from the primitives: move i of chain c.
+    # Rebuild the draws of sweep 0 from raw Philox blocks: move i of chain c
+    # picks with words (0, 1) or (2, 3) of the pick block of index i // 2 (53
+    # bits), and takes its accept uniform from the low bytes of words 0, 1, 2
+    # of its displacement block (index i).
     N = spec.polymers[0].length
     pick = np.empty(n)
     gauss = np.empty((n, 3), np.float32)
@@ -44,10 +47,13 @@ def test_injected_sweep_matches_streams_when_fed_the_same_numbers():
     for c in range(spec.polymers[0].count):
         gauss[c * N:(c + 1) * N] = orc.normals_f(3, 0, c, 0, N)
         for i in range(N):
-            w = [int(v) for v in orc.philox((c, 0, (i << 2) | 0, 0), (3, 0))]
-            m = ((w[0] >> 5) << 26) | (w[1] >> 6)
+            w = [int(v) for v in orc.philox((c, 0, ((i >> 1) << 2) | 0, 0), (3, 0))]
+            a, b = (w[2], w[3]) if i & 1 else (w[0], w[1])
+            m = ((a >> 5) << 26) | (b >> 6)
             pick[c * N + i] = (m + 1) * 2.0 ** -53
-            acc_u[c * N + i] = np.float32(((w[2] >> 8) + 1) * 2.0 ** -24)
+            d = [int(v) for v in orc.philox((c, 0, (i << 2) | 1, 0), (3, 0))]
+            bits = (d[0] & 0xFF) | ((d[1] & 0xFF) << 8) | ((d[2] & 0xFF) << 16)
+            acc_u[c * N + i] = np.float32((bits + 1) * 2.0 ** -24)
     p1 = pos.copy()
     a1 = o.sweep(p1, om, 0)
     p2 = pos.copy()
