@@ -7,11 +7,16 @@ fA=0.3, 4,194,304 chains (268,435,456 beads), 256^3 grid, chiN=25, kappaN=70,
 uint16 counts, fp32 coordinates; synthetic Gaussian-walk initial state.
 
 Prints one JSON line.  `value` is bead-moves per second over the timed steps
-(positions, counts and fields resident in HBM throughout).  `roofline` is
-for the dominant kernel (the sweep), from HIP events on the engine's stream;
-`passes` gives the same for the gridding and omega passes.  `cpu_baseline` is
-the CPU oracle (the restated reference move logic, OpenMP over chains) timed
-on a bounded sample of the same workload on this host.
+(positions, counts and fields resident in HBM throughout); the timed window
+starts at a spatial ordering (every 25 steps) and holds ceil(steps / 25) of
+them.  `roofline` is for the dominant kernel (the sweep: move schedules +
+moves), from HIP events on the engine's stream, against the scope table's
+algorithmic bytes, with the counter-based VALU issue figure beside it;
+`passes` gives the gridding and omega passes.  `small` is the 262,144-bead
+trial-move melt's whole run (10,000 steps), `largest` the 1.0e9-bead blend,
+`batched` and `served` the many-system forms.  `cpu_baseline` is the CPU
+oracle (the restated reference move, gridding and omega logic, OpenMP) built
+for and timed on this host.
 """
 import argparse
 import json
@@ -19,6 +24,8 @@ import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -35,59 +42,147 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(spec, opts, target_s=15.0):
-    """The oracle's step on all host threads: its sweep timed on the first
-    chains of the workload, its gridding and omega update on the whole system."""
-    import numpy as np
+def host_info():
+    """The host the CPU baseline runs on: logical CPUs visible to this process,
+    the CPU model (lscpu, else /proc/cpuinfo) and the OpenMP threads used."""
+    import subprocess
+    info = {"nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count()}
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core"):
+                info[k.strip()] = v.strip()
+        model = info.get("Model name")
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        info["Model name"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return info
 
-    from new_arch_amd.abi import SystemSpec
+
+def native_oracle():
+    """Compile the oracle for this host (-O3 -march=native -fopenmp, FP
+    contraction off, as the checker is built) into a temporary directory and
+    point the oracle binding at it; fall back to the prebuilt portable library
+    (-march=x86-64-v3) if the compiler is unavailable.  Returns the build used."""
+    import subprocess
+    import tempfile
     from oracle import oracle as orc
 
-    N = spec.polymers[0].length
+    src = os.path.join(ROOT, "oracle", "scmf_oracle.cpp")
+    out = os.path.join(tempfile.mkdtemp(prefix="scmf_oracle_"), "liboracle_native.so")
+    cmd = ["g++", "-O3", "-march=native", "-std=c++17", "-fPIC", "-shared", "-fopenmp",
+           "-ffp-contract=off", "-fno-fast-math", "-o", out, src]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+        if r.returncode == 0 and orc._lib is None:
+            orc.LIB_PATH = out
+            return "g++ -O3 -march=native -fopenmp -ffp-contract=off (built on this host)"
+        log(f"native oracle build failed: {r.stderr[-300:]}")
+    except (OSError, subprocess.SubprocessError) as e:
+        log(f"native oracle build failed: {e}")
+    return "g++ -O3 -march=x86-64-v3 -fopenmp -ffp-contract=off (prebuilt)"
 
-    def sub(chains):
-        p = spec.polymers[0]
-        return SystemSpec(n_types=spec.n_types, xn=spec.xn, L=spec.L, n=spec.n, dtNxi=spec.dtNxi,
-                          nref=spec.nref, polymers=[type(p)(length=N, count=chains,
-                                                            monotype=p.monotype)])
 
-    om = np.zeros(spec.n_types * spec.n_cells, np.float32)
-    chains = 4096
-    o = orc.Oracle(sub(chains), opts)
-    pos = o.init_coords()
-    t = time.perf_counter()
-    o.sweep(pos, om, 0)
-    dt = time.perf_counter() - t
-    per_chain_sweep = dt / chains
-    chains = int(max(4096, min(spec.n_chains, target_s / (2 * per_chain_sweep))))
-    sweeps = int(max(2, min(20, target_s / (chains * per_chain_sweep))))
-    o = orc.Oracle(sub(chains), opts)
-    pos = o.init_coords()
-    t = time.perf_counter()
-    for s in range(sweeps):
-        o.sweep(pos, om, s)
-    dt = time.perf_counter() - t
-    moves = chains * N * sweeps
-    sweep_rate = moves / dt
-    # The gridding and the omega update of the whole system, once each.
+def cpu_step_figures(spec, opts, sweep_target_s=12.0, max_sweeps=20):
+    """The oracle's step (sweep + gridding + omega) on one named system: the
+    gridding and the omega update of the whole system, then the sweep with the
+    omega just computed, over every chain if that fits the time budget, else
+    over the first chains (a bounded sample) scaled to the whole system."""
+    from oracle import oracle as orc
+
     o = orc.Oracle(spec, opts)
     pos = o.init_coords()
     t = time.perf_counter()
     counts = o.grid(pos)
     t_grid = time.perf_counter() - t
-    del pos
     t = time.perf_counter()
-    o.omega(counts)
+    om = o.omega(counts).astype(np.float32)
     t_omega = time.perf_counter() - t
+    N = max(p.length for p in spec.polymers)
+    probe = min(spec.n_chains, 4096)
+    t = time.perf_counter()
+    o.sweep(pos, om, 0, 0, probe)
+    per_chain = (time.perf_counter() - t) / probe
+    chains = int(max(probe, min(spec.n_chains, sweep_target_s / per_chain)))
+    sweeps = int(max(1, min(max_sweeps, sweep_target_s / (chains * per_chain))))
+    t = time.perf_counter()
+    for k in range(sweeps):
+        o.sweep(pos, om, 1 + k, 0, chains)
+    dt = time.perf_counter() - t
+    beads_swept = sum(p.count * p.length for p in spec.polymers) * chains / spec.n_chains
+    sweep_rate = beads_swept * sweeps / dt
     B = spec.n_beads
     step_s = B / sweep_rate + t_grid + t_omega
-    return {"value": B / step_s, "unit": "bead-moves/s", "cores": orc.threads(), "kind": "port",
-            "sweep_bead_moves_per_s": sweep_rate, "grid_s": round(t_grid, 3),
-            "omega_s": round(t_omega, 4),
-            "sample": f"one step = sweep + gridding + omega: the oracle's smart-MC sweep of the "
-                      f"first {chains} chains ({chains * N} beads) x {sweeps} sweeps ({dt:.1f} s) "
-                      f"scaled to {B} beads, plus its gridding ({t_grid:.2f} s) and omega update "
-                      f"({t_omega:.3f} s) of the whole system"}
+    return {"bead_moves_per_s": B / step_s, "sweep_bead_moves_per_s": sweep_rate,
+            "grid_s": round(t_grid, 4), "omega_s": round(t_omega, 5),
+            "sweep_chains": chains, "sweeps": sweeps, "sweep_s": round(dt, 2),
+            "step_s": round(step_s, 4)}
+
+
+def cpu_full_run(spec, opts, steps):
+    """The oracle's whole step loop (test.cpp:37-43 with omega every sweep) for
+    `steps` steps on the whole system: bead-moves/s of the full run."""
+    from oracle import oracle as orc
+
+    o = orc.Oracle(spec, opts)
+    pos = o.init_coords()
+    om = o.omega(o.grid(pos)).astype(np.float32)
+    t = time.perf_counter()
+    for k in range(steps):
+        o.sweep(pos, om, k)
+        om = o.omega(o.grid(pos)).astype(np.float32)
+    dt = time.perf_counter() - t
+    return {"bead_moves_per_s": spec.n_beads * steps / dt, "steps": steps,
+            "seconds": round(dt, 2)}
+
+
+def cpu_baseline(spec, opts, small_steps=10000, with_largest=True):
+    """The CPU oracle (the restated reference move, gridding and omega logic,
+    OpenMP over chains / private histograms / cells) on this host: the headline
+    system's step from a bounded sweep sample plus whole-system gridding and
+    omega; the small melt's whole run; the largest system's step."""
+    from new_arch_amd import workloads as W
+    from new_arch_amd.abi import make_opts
+    from oracle import oracle as orc
+
+    build = native_oracle()
+    hi = host_info()
+    log(f"cpu baseline: {build}; {orc.threads()} threads; {hi.get('Model name')}")
+    head = cpu_step_figures(spec, opts)
+    log(f"cpu headline: {head}")
+    small = W.small_diblock_melt()
+    sopts = make_opts(seed=0, move="trial", trial_delta=0.1, omega_every_step=True,
+                      density_bits=16)
+    small_run = cpu_full_run(small, sopts, small_steps)
+    log(f"cpu small melt, {small_steps} steps: {small_run}")
+    largest = None
+    if with_largest:
+        lg = W.largest_blend()
+        largest = cpu_step_figures(lg, make_opts(seed=0, move="smart", omega_every_step=True,
+                                                 density_bits=32), sweep_target_s=8.0,
+                                   max_sweeps=1)
+        log(f"cpu largest: {largest}")
+    N = spec.polymers[0].length
+    return {"value": head["bead_moves_per_s"], "unit": "bead-moves/s", "cores": orc.threads(),
+            "kind": "port", "build": build, "host": hi,
+            "sample": f"one step of the headline system = the oracle's smart-MC sweep of the first "
+                      f"{head['sweep_chains']} chains x {head['sweeps']} sweeps ({head['sweep_s']} s, "
+                      f"against the omega of the system's own counts) scaled to {spec.n_beads} "
+                      f"beads, plus its gridding ({head['grid_s']} s) and omega update "
+                      f"({head['omega_s']} s) of the whole system",
+            "headline": head,
+            "small_diblock_melt_full_run": small_run,
+            "largest_blend_step": largest}
 
 
 def batched_vs_plain(steps, warmup, n_systems=64):
@@ -213,9 +308,45 @@ def served(n_jobs, rate, seed=0, steps_per_tick=25):
                        "cells), chiN 10-50, 500-5000 sweeps, 128 beads/Re^3, uint16"}
 
 
-def measure_system(workload, steps, warmup, with_host_copy=False):
-    """Create the named system, autotune the sweep shape, run `warmup` steps,
-    then time `steps` steps (sweep + gridding + omega every sweep)."""
+def profile_json(workload, kind):
+    """Per-kernel figures of a committed rocprofv3 run of this workload
+    (profiles/<workload>_<kind>.json), or {} if there is none."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{kind}.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def issue_figures(workload, kernels, B):
+    """VALU issue of the sweep from the committed counter run: SIMD issue
+    cycles 4 (SQ_INSTS_VALU - SQ_ACTIVE_INST_VALU2) / SIMDs against the
+    elapsed GRBM_GUI_ACTIVE cycles per XCD (the model the calibration kernels
+    of libscmf_calib.so confirm within 94-96% for VALU-saturated kernels)."""
+    prof = profile_json(workload, "valu").get("per_launch", {})
+    cal = profile_json("valu", "calibration")
+    if not all(k in prof for k in kernels):
+        return None
+    insts = sum(prof[k]["SQ_INSTS_VALU"] for k in kernels)
+    dual = sum(prof[k]["SQ_ACTIVE_INST_VALU2"] for k in kernels)
+    cycles = sum(prof[k]["GRBM_GUI_ACTIVE"] for k in kernels) / 8.0
+    simds = 1024
+    busy = 4.0 * (insts - dual) / simds / cycles
+    sat = cal.get("saturated_issue_fraction")
+    return {"bound": "valu-issue", "valu_wave_instructions": insts,
+            "dual_issued_quad_cycles": dual, "elapsed_cycles": cycles,
+            "issue_busy": round(busy, 4),
+            "of_saturated": round(busy / sat, 4) if sat else None,
+            "saturated_issue_fraction": sat,
+            "wave_instructions_per_move": round(insts / B, 3),
+            "source": f"profiles/{workload}_valu.json, profiles/valu_calibration.json"}
+
+
+def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True):
+    """Create the named system, autotune the sweep shape, run `warmup` steps
+    (plus the untimed steps up to the next spatial ordering, so the timed
+    window starts with one and holds ceil(steps / 25) of them), then time
+    `steps` steps (sweep + gridding + omega every sweep)."""
     from new_arch_amd import scmf
     from new_arch_amd import workloads as W
     from new_arch_amd.abi import make_opts
@@ -231,27 +362,51 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
     g.update_omega()
     g.sync()
     log(f"created in {time.perf_counter() - t:.1f} s")
-    lanes, tuned_ms = g.autotune(2)
+    lanes, tuned_ms = g.autotune(2) if autotune else (0, 0.0)
     log(f"autotune: {lanes} lanes per chain, {tuned_ms:.3f} ms per sweep")
-    for i in range(warmup):
-        g.step(i + 1)
+    period = 25 if spec.n_beads >= (1 << 23) else 0   # automatic ordering period
+    step_no = 0
+    for _ in range(warmup):
+        step_no += 1
+        g.step(step_no)
+    if period:
+        while g.stats().sweeps % period:
+            step_no += 1
+            g.step(step_no)
     g.sync()
-    log(f"warmup {warmup} steps done")
+    log(f"warmup {step_no} steps done")
     g.enable_timing(True)
     g.reset_timing()
     g.reset_stats()
     g.sync()
-    cell_grids0 = g.stats().cell_grids
-    sorts0 = g.stats().sorts
+    st0 = g.stats()
     t0 = time.perf_counter()
     for i in range(steps):
-        g.step(warmup + i + 1)
+        g.step(step_no + i + 1)
     g.sync()
     t1 = time.perf_counter()
+    step_no += steps
     tm = g.timing()
     st = g.stats()
-    # every timed gridding read the bead cells the sweep stored
-    from_cells = st.cell_grids - cell_grids0 == steps
+    from_cells = st.cell_grids - st0.cell_grids > 0
+    ordering = None
+    if period:
+        # one ordering and the gridding after it (which bins positions: the
+        # gridding of the scope table, 12 B a bead) timed on their own
+        g.enable_timing(False)
+        g.sync()
+        t = time.perf_counter()
+        g.sort_chains()
+        g.sync()
+        t_sort = time.perf_counter() - t
+        g.enable_timing(True)
+        g.reset_timing()
+        g.grid_density()
+        g.sync()
+        t_bin = g.timing().grid_ms
+        ordering = {"period_steps": period, "in_timed_steps": int(st.sorts - st0.sorts),
+                    "ordering_ms": round(t_sort * 1e3, 3),
+                    "gridding_from_positions_ms": round(t_bin, 4)}
     host = None
     if with_host_copy:
         # The rate a caller would see if it copied the positions out and back
@@ -312,33 +467,27 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
     g.close()
     log(f"{workload}: timed {steps} steps in {t1 - t0:.3f} s; sweep {tm.sweep_ms:.3f} ms, "
         f"grid {tm.grid_ms:.3f} ms, omega {tm.omega_ms:.3f} ms; acceptance "
-        f"{st.accepts / max(st.moves, 1):.3f}; grid fallback {st.grid_fallback}")
+        f"{st.accepts / max(st.moves, 1):.3f}; grid fallback {st.grid_fallback}; ordering {ordering}")
 
     B, TC = spec.n_beads, spec.n_types * spec.n_cells
     s = 2 if bits == 16 else 4
-    # positions read + written, omega read once (+ the bead cells written
-    # for the gridding); the gridding reads the cells (4 B) or the positions
-    # (12 B) and writes the counts
-    sweep_bytes = 24 * B + 4 * TC + (4 * B if from_cells else 0)
-    # chains of at most 255 beads: k_schedule writes each chain's schedule
-    # record (2 nmax + 3 bytes) and the sweep reads it back
     nmax = max(p.length for p in spec.polymers)
-    pre = nmax <= 255 and lanes >= 2
-    if pre:
-        sweep_bytes += 2 * (2 * nmax + 3) * spec.n_chains
-    grid_bytes = (4 if from_cells else 12) * B + s * TC
+    pre = nmax <= 255 and lanes >= 2 and spec.n_chains >= 65536
+    # algorithmic bytes of the scope table: positions read + written, omega read once
+    sweep_bytes = 24 * B + 4 * TC
+    # bytes the implementation moves beyond them: the bead cells the sweep stores
+    # for the next gridding, the move-schedule records written and read back
+    impl = {"cells_written": 4 * B if from_cells else 0,
+            "schedule_records": 2 * (2 * nmax + 3) * spec.n_chains if pre else 0}
+    grid_bytes_cells = 4 * B + s * TC   # the pass as run: cells read, counts written
     omega_bytes = s * TC + 4 * TC       # counts read, fp32 omega written
     tiled = spec.n_beads >= (1 << 23)
     sweep_kernel = {-1: "k_sweep_global", 1: "k_sweep_lds"}.get(lanes, f"k_sweep_sched<{lanes}>")
 
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     # of this workload (calibration and commands in the profile summary).
-    measured = {}
-    tpath = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            measured = json.load(f).get("bytes_per_launch", {})
-    byte_t = "unsigned char" if spec.polymers and max(p.length for p in spec.polymers) <= 255 else "unsigned short"
+    measured = profile_json(workload, "traffic").get("bytes_per_launch", {})
+    byte_t = "unsigned char" if nmax <= 255 else "unsigned short"
     sweep_keys = ([f"k_sweep_sched<{lanes}, {byte_t}>"] if lanes > 1 else [sweep_kernel]) + \
         (["k_schedule"] if pre else [])
     omega_key = (f"k_omega4<{spec.n_types}, {bits}>" if (spec.n_cells % 4 == 0)
@@ -363,6 +512,34 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
             out["traffic_source"] = f"profiles/{workload}_kernel_stats.md"
         return out
 
+    passes = {"grid": dict(roof(grid_bytes_cells, tm.grid_ms,
+                                traffic_of(grid_keys) if tiled else None),
+                           kernel=("k_zero_counts+k_grid_tiles" + ("(cells)+" if from_cells else "+")
+                                   + ("k_tiles_sum_pair" if spec.n_types == 2 else "k_tiles_sum"))
+                           if tiled else "k_zero_counts+k_grid",
+                           bytes_note="cells read (4 B a bead) + counts written" if from_cells
+                           else "positions read (12 B a bead) + counts written"),
+              "omega": dict(roof(omega_bytes, tm.omega_ms, traffic_of([omega_key])),
+                            kernel=omega_key)}
+    if not from_cells:
+        passes["grid"]["algorithmic_bytes"] = 12 * B + s * TC
+        passes["grid"] = dict(roof(12 * B + s * TC, tm.grid_ms, passes["grid"]["traffic"]),
+                              kernel=passes["grid"]["kernel"])
+    if ordering:
+        # the gridding of the scope table: binning positions, 12 B a bead
+        passes["grid_from_positions"] = dict(
+            roof(12 * B + s * TC, ordering["gridding_from_positions_ms"], None),
+            kernel=f"k_zero_counts+k_grid_tiles(positions)+"
+                   f"{'k_tiles_sum_pair' if spec.n_types == 2 else 'k_tiles_sum'}",
+            when="after each spatial ordering")
+    dtNxi, nref = spec.dtNxi, spec.nref
+    A = dtNxi / nref
+    roofline = dict(roof(sweep_bytes, tm.sweep_ms, traffic_of(sweep_keys)),
+                    kernel=("k_schedule+" if pre else "") + sweep_kernel,
+                    implementation_bytes=impl)
+    issue = issue_figures(workload, sweep_keys, B)
+    if issue:
+        roofline["issue"] = issue
     return {
         "_spec": spec, "_opts": opts,
         "workload": workload,
@@ -373,21 +550,22 @@ def measure_system(workload, steps, warmup, with_host_copy=False):
         "config": {"workload": workload, "beads": B, "chains": spec.n_chains,
                    "N": [p.length for p in spec.polymers], "grid": list(spec.n),
                    "box_Re": list(spec.L), "xn": spec.xn, "move": move,
+                   "dtNxi": dtNxi, "Nref": nref,
+                   "A_Re2": A if move == "smart" else None,
+                   "R_Re": math.sqrt(2 * A) if move == "smart" else None,
+                   "trial_delta_Re": 0.1 if move == "trial" else None,
+                   "step_parameter": "smart-MC drift A = dtNxi / Nref and noise R = sqrt(2 A) "
+                                     "(init_config.cu:804-809); the configuration's step 0.05 "
+                                     "is read as dtNxi" if move == "smart" else
+                                     "uniform displacement of +-0.1 Re per component",
                    "density_bits": bits, "omega": "every sweep", "parallelism": "1 GPU",
                    "lanes_per_chain": lanes},
-        "roofline": dict(roof(sweep_bytes, tm.sweep_ms, traffic_of(sweep_keys)),
-                         kernel=("k_schedule+" if pre else "") + sweep_kernel),
-        "passes": {"grid": dict(roof(grid_bytes, tm.grid_ms,
-                                     traffic_of(grid_keys) if tiled else None),
-                                kernel=("k_zero_counts+k_grid_tiles" + ("(cells)+" if from_cells else "+")
-                                        + ("k_tiles_sum_pair" if spec.n_types == 2 else "k_tiles_sum"))
-                                if tiled else "k_zero_counts+k_grid"),
-                   "omega": dict(roof(omega_bytes, tm.omega_ms, traffic_of([omega_key])),
-                                 kernel=omega_key)},
+        "roofline": roofline,
+        "passes": passes,
         "acceptance": st.accepts / max(st.moves, 1),
         "grid_fallback_beads": int(st.grid_fallback),
-        # spatial orderings inside the timed steps (one every 25 sweeps)
-        "orderings_timed": int(st.sorts - sorts0),
+        "orderings_timed": int(st.sorts - st0.sorts),
+        "ordering": ordering,
         "pcie_inclusive": host,
         "analysis": analysis,
     }
@@ -403,6 +581,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-largest", action="store_true")
+    ap.add_argument("--no-small", action="store_true")
+    ap.add_argument("--small-steps", type=int, default=10000)  # the small melt's whole run
     ap.add_argument("--served-jobs", type=int, default=200)
     ap.add_argument("--served-rate", type=float, default=6.0)
     args = ap.parse_args()
@@ -431,12 +611,18 @@ def main():
         "passes": m["passes"],
         "acceptance": m["acceptance"],
         "orderings_timed": m["orderings_timed"],
+        "ordering": m["ordering"],
         "pcie_inclusive": m["pcie_inclusive"],
         "analysis": m["analysis"],
     }
+    if not args.no_small and args.workload != "small_diblock_melt":
+        log("small system")
+        sm = measure_system("small_diblock_melt", steps=args.small_steps, warmup=100)
+        sm.pop("_spec"), sm.pop("_opts")
+        out["small"] = sm
     if not args.no_largest and args.workload != "largest_blend":
         log("largest system")
-        lg = measure_system("largest_blend", steps=5, warmup=1)
+        lg = measure_system("largest_blend", steps=25, warmup=1)
         lg.pop("_spec"), lg.pop("_opts")
         out["largest"] = lg
     if not args.no_batched:
@@ -447,7 +633,7 @@ def main():
         out["served"] = served(args.served_jobs, args.served_rate)
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(spec, opts)
+        out["cpu_baseline"] = cpu_baseline(spec, opts, with_largest=not args.no_largest)
     print(json.dumps(out), flush=True)
 
 

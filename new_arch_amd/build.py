@@ -1,4 +1,5 @@
-"""Build recipe for the native parts: libscmf.so + scmf_run (HIP, gfx950).
+"""Build recipe for the native parts: libscmf.so + scmf_run (HIP, gfx950), and
+libscmf_calib.so (kernels of known VALU instruction count, for measurement).
 
 Every HIP translation unit is compiled with FP contraction off (results must
 match the CPU oracle bit for bit) and with packed-FP32 instructions disabled.
@@ -14,6 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libscmf.so")
 RUNNER = os.path.join(LIBDIR, "scmf_run")
+CALIB = os.path.join(LIBDIR, "libscmf_calib.so")  # VALU-rate calibration kernels (measurement)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -37,6 +39,10 @@ def build(verbose=True):
           os.path.join(CSRC, "scmf_server.cpp"), "-L/opt/rocm/lib", "-lhipfft",
           "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
+    tmp = CALIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, "-shared", "-o", tmp,
+          os.path.join(CSRC, "scmf_calib.hip")])
+    os.replace(tmp, CALIB)
     tmp = RUNNER + ".tmp"
     _run([HIPCC, *COMMON, "-o", tmp, os.path.join(CSRC, "scmf_main.cpp"),
           f"-L{LIBDIR}", "-lscmf", "-Wl,-rpath,$ORIGIN"])
