@@ -153,6 +153,18 @@ int scmf_step(scmf_sys* sys, int32_t step);
  * test.cpp:51-58): copy a device array to host memory. */
 int scmf_copy_out(scmf_sys* sys, int32_t what, void* host, size_t bytes);
 
+/* No counterpart in the reference.  scmf_copy_out for callers that cannot
+   allocate device memory (code under HIP graph capture, serving loops, hosts
+   that forbid allocation): the same copies, staged through a fixed 16 MiB
+   device array instead of buffers made on demand (scmf_copy_out allocates a
+   position staging buffer on first use and a widening buffer on every
+   uint16 copy).  Positions of at most 1,398,101 beads and uint16 counts of
+   at most 4,194,304 types x cells; larger systems are refused with
+   SCMF_E_ARG (use scmf_copy_out).  uint32 counts and omega are copied
+   directly.  Calls in one process take a lock, since every call on a GPU
+   shares that GPU's array (which keeps the last call's contents). */
+int scmf_copy_out_noalloc(scmf_sys* sys, int32_t what, void* host, size_t bytes);
+
 /* coord.dat restart (init_config.cu:930-987) and parity injection. */
 int scmf_copy_in_pos(scmf_sys* sys, const float* xyz, size_t n_beads);
 int scmf_copy_in_omega(scmf_sys* sys, const float* omega, size_t n_values);

@@ -1025,6 +1025,11 @@ __global__ void __launch_bounds__(BLOCK) k_widen(const uint32_t* __restrict__ wo
     out[i] = load_count<BITS>(words, (uint32_t)i);
 }
 
+// Staging of scmf_copy_out_noalloc: a fixed device array, no allocation.
+// Every GPU the library runs on holds its own copy until the process ends.
+constexpr int64_t NOALLOC_WORDS = (int64_t)1 << 22;  // 16 MiB
+static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];
+
 }  // namespace
 
 // ============================================================== host side
@@ -1103,7 +1108,7 @@ struct scmf_sys {
   float* x2 = nullptr;
   float* y2 = nullptr;
   float* z2 = nullptr;
-  uint16_t* scratch = nullptr;  // window images of the tiled gridding
+  uint16_t* images = nullptr;  // window images of the tiled gridding
   uint32_t* cells = nullptr;    // packed bead cells written by the sweep (tiled systems)
   uint8_t* sched = nullptr;     // move schedules made by k_schedule (chains of <= 255 beads)
   bool pre_sched = false;       // the scheduled sweep shapes read them
@@ -1405,7 +1410,7 @@ int launch_sweep(scmf_sys* s) {
 // typical chain from its middle bead (0.75 Re scaled by sqrt((N-1)/(Nref-1))
 // for the bead-weighted median length N); beads beyond it are counted with
 // global atomics.  Among the tilings whose LDS window fits, the one with the
-// least window area per interior cell wins (that ratio sets the scratch
+// least window area per interior cell wins (that ratio sets the image
 // traffic), with a penalty for one workgroup per CU and for too few tiles to
 // fill the GPU.
 size_t tile_lds_cap() { return (size_t)160 << 10; }
@@ -1550,7 +1555,7 @@ int alloc_sort_buffers(scmf_sys* s) {
       (rc = A((void**)&s->x2, sizeof(float) * padded(B))) ||
       (rc = A((void**)&s->y2, sizeof(float) * padded(B))) ||
       (rc = A((void**)&s->z2, sizeof(float) * padded(B))) ||
-      (rc = A((void**)&s->scratch,
+      (rc = A((void**)&s->images,
               sizeof(uint16_t) * ((size_t)s->geo.ntiles * s->geo.slot_elems + 64))))
     return rc;
   // the sweep stores bead cells for the next gridding where 10 bits per axis hold them
@@ -1562,7 +1567,7 @@ int alloc_sort_buffers(scmf_sys* s) {
   }
   // the window sums read absent contributions from the element after the
   // last image, which nothing writes
-  HIP_TRY(hipMemsetAsync(s->scratch + (size_t)s->geo.ntiles * s->geo.slot_elems, 0,
+  HIP_TRY(hipMemsetAsync(s->images + (size_t)s->geo.ntiles * s->geo.slot_elems, 0,
                          sizeof(uint16_t) * 64, s->stream));
   if (s->tile_lds + MONO_LDS + 128 > 65536) {
     for (int v = 0; v < 16; ++v) {
@@ -1620,19 +1625,19 @@ int launch_tiled_grid(scmf_sys* s, hipStream_t st, const KParams* Ps, int sys, b
   const void* bin = grid_tiles_kernel(use_cells, s->opts.density_bits == 32, s->kp.pow2 != 0,
                                       s->T == 2);
   const uint32_t* tstart = s->tstart;
-  void* args[] = {(void*)&Ps, (void*)&sys, (void*)&s->geo, (void*)&tstart, (void*)&s->scratch,
+  void* args[] = {(void*)&Ps, (void*)&sys, (void*)&s->geo, (void*)&tstart, (void*)&s->images,
                   (void*)&s->d_fallback};
   HIP_TRY(hipLaunchKernel(bin, tiles, dim3(GRID_THREADS), args, s->tile_lds, st));
   if (s->T == 2) {
     auto sum = s->opts.density_bits == 32 ? k_tiles_sum_pair<32> : k_tiles_sum_pair<16>;
     hipLaunchKernelGGL(sum, tiles, dim3(SUM_THREADS), 0, st, Ps, sys, s->geo,
-                       (const uint32_t*)s->scratch);
+                       (const uint32_t*)s->images);
     HIP_TRY(hipGetLastError());
     return SCMF_OK;
   }
   auto sum = s->opts.density_bits == 32 ? k_tiles_sum<32> : k_tiles_sum<16>;
   hipLaunchKernelGGL(sum, tiles, dim3(SUM_THREADS), 0, st, Ps, sys, s->geo,
-                     (const uint16_t*)s->scratch);
+                     (const uint16_t*)s->images);
   HIP_TRY(hipGetLastError());
   return SCMF_OK;
 }
@@ -1644,7 +1649,7 @@ int free_all(scmf_sys* s) {
                   (void*)s->staging, (void*)s->slot_chain, (void*)s->chain_slot,
                   (void*)s->slot_chain2, (void*)s->btype, (void*)s->keys, (void*)s->tcount,
                   (void*)s->tstart, (void*)s->tfill, (void*)s->new_slot, (void*)s->x2,
-                  (void*)s->y2, (void*)s->z2, (void*)s->scratch, (void*)s->d_fallback,
+                  (void*)s->y2, (void*)s->z2, (void*)s->images, (void*)s->d_fallback,
                   (void*)s->d_kp, (void*)s->d_ctr, (void*)s->fft_buf, (void*)s->d_sq_sum,
                   (void*)s->d_sq_cnt, (void*)s->cells, (void*)s->sched})
     if (p) (void)hipFree(p);
@@ -2159,6 +2164,88 @@ int scmf_copy_out(scmf_sys* s, int32_t what, void* host, size_t bytes) {
       (void)hipFree(wide);
       HIP_TRY(e1);
       HIP_TRY(e2);
+    }
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SCMF_OK;
+  }
+  if (what == SCMF_OMEGA) {
+    if (bytes < sizeof(float) * (size_t)TC) return fail(SCMF_E_ARG, "buffer too small");
+    HIP_TRY(hipMemcpyAsync(host, s->omega, sizeof(float) * TC, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SCMF_OK;
+  }
+  return fail(SCMF_E_ARG, "unknown copy_out selector");
+}
+
+}  // extern "C"
+
+namespace {
+
+// The device address of `scratch` on each GPU, looked up once per device with
+// that device current (the array's name is a host-side stand-in, not a
+// device address), and the lock that serialises the calls using it.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+uint32_t* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mu;
+
+int scratch_on(int device, uint32_t** out) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return fail(SCMF_E_NODEV, "device index out of range");
+  if (!g_scratch_ptr[device]) {
+    int cur = 0;
+    HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) HIP_TRY(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(SCMF_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = (uint32_t*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return SCMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int scmf_copy_out_noalloc(scmf_sys* s, int32_t what, void* host, size_t bytes) {
+  if (!s || !host) return fail(SCMF_E_ARG, "null argument");
+  const int64_t TC = (int64_t)s->T * s->n_cells;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (what == SCMF_POS) {
+    if (bytes < sizeof(float) * 3 * (size_t)s->n_beads) return fail(SCMF_E_ARG, "buffer too small");
+    if (3 * s->n_beads > NOALLOC_WORDS)
+      return fail(SCMF_E_ARG, "scmf_copy_out_noalloc: positions of at most " +
+                                  std::to_string(NOALLOC_WORDS / 3) + " beads (the system has " +
+                                  std::to_string(s->n_beads) + "); use scmf_copy_out");
+    uint32_t* ptr = nullptr;
+    int rc = scratch_on(s->device, &ptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_gather_ref, dim3(grid_strided(s->n_beads)), dim3(BLOCK), 0, s->stream,
+                       s->kp, s->x, s->y, s->z, (float*)ptr, (int64_t)0, s->n_beads);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(host, ptr, sizeof(float) * 3 * s->n_beads, hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return SCMF_OK;
+  }
+  if (what == SCMF_DENSITY) {
+    if (bytes < sizeof(uint32_t) * (size_t)TC) return fail(SCMF_E_ARG, "buffer too small");
+    if (s->opts.density_bits == 32) {
+      HIP_TRY(hipMemcpyAsync(host, s->counts, sizeof(uint32_t) * TC, hipMemcpyDeviceToHost, s->stream));
+    } else {
+      if (TC > NOALLOC_WORDS)
+        return fail(SCMF_E_ARG, "scmf_copy_out_noalloc: uint16 counts of at most " +
+                                    std::to_string(NOALLOC_WORDS) + " (types x cells; the system has " +
+                                    std::to_string(TC) + "); use scmf_copy_out");
+      uint32_t* ptr = nullptr;
+      int rc = scratch_on(s->device, &ptr);
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_widen<16>, dim3(grid_strided(TC)), dim3(BLOCK), 0, s->stream, s->counts,
+                         ptr, TC);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(host, ptr, sizeof(uint32_t) * TC, hipMemcpyDeviceToHost, s->stream));
     }
     HIP_TRY(hipStreamSynchronize(s->stream));
     return SCMF_OK;

@@ -42,7 +42,7 @@ struct TileGeo {
   int win_cells;      // W0 W1 W2
   int win_words;      // T win_cells / 2, rounded up (uint16 pairs)
   int win_words_padded;  // win_words rounded up to a multiple of 4
-  int slot_elems;     // uint16 elements of one window image in scratch
+  int slot_elems;     // uint16 elements of one window image in the image buffer
   int reg_off[27];    // element offset of region rx + 3 ry + 9 rz in an image
 };
 
@@ -300,7 +300,7 @@ template <int BITS, bool POW2, bool CELLS, bool PAIR>
 __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __restrict__ Ps, int sys,
                                                              TileGeo g,
                                                              const uint32_t* __restrict__ tile_start,
-                                                             uint16_t* __restrict__ scratch,
+                                                             uint16_t* __restrict__ images,
                                                              unsigned long long* __restrict__ fallback) {
   extern __shared__ __align__(16) uint32_t win[];
   // types of an architecture's monomers (with PAIR, 16 x type: the shift of
@@ -559,13 +559,13 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   if (threadIdx.x == 0 && missed_s) atomicAdd(fallback, (unsigned long long)missed_s);
   if (PAIR) {
     // the window verbatim, 16 bytes a lane (images are 128-byte aligned)
-    uint4* __restrict__ dst = (uint4*)(scratch + (int64_t)t * g.slot_elems);
+    uint4* __restrict__ dst = (uint4*)(images + (int64_t)t * g.slot_elems);
     for (int w4 = threadIdx.x; 4 * w4 < g.win_words_padded; w4 += GRID_THREADS)
       dst[w4] = ((const uint4*)win)[w4];
     return;
   }
   // store the window as 27 regions
-  uint16_t* __restrict__ img = scratch + (int64_t)t * g.slot_elems;
+  uint16_t* __restrict__ img = images + (int64_t)t * g.slot_elems;
   const int sz3[3] = {sx, sy, sz};
   for (int r = 0; r < 27; ++r) {
     const int rx = r % 3, ry = (r / 3) % 3, rz = r / 9;
@@ -630,7 +630,7 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 template <int BITS>
 __global__ void __launch_bounds__(SUM_THREADS) k_tiles_sum(const KParams* __restrict__ Ps, int sys,
                                                    TileGeo g,
-                                                   const uint16_t* __restrict__ scratch) {
+                                                   const uint16_t* __restrict__ images) {
   KPC& kp = param(Ps, (uint32_t)sys);
   SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
   SCMF_GLOBAL uint16_t* __restrict__ counts16 = gptr((uint16_t*)kp.counts);
@@ -686,7 +686,7 @@ __global__ void __launch_bounds__(SUM_THREADS) k_tiles_sum(const KParams* __rest
     }
     uint32_t sum = BITS == 32 ? counts[cell] : (uint32_t)counts16[cell];
 #pragma unroll
-    for (int c = 0; c < 8; ++c) sum += scratch[idx[c]];
+    for (int c = 0; c < 8; ++c) sum += images[idx[c]];
     if (BITS == 32) {
       counts[cell] = sum;
     } else {

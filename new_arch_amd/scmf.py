@@ -42,6 +42,7 @@ def lib():
                  "scmf_reset_timing", "scmf_sync", "scmf_sort_chains"):
         getattr(L, name).argtypes = [vp]
     L.scmf_copy_out.argtypes = [vp, C.c_int32, vp, C.c_size_t]
+    L.scmf_copy_out_noalloc.argtypes = [vp, C.c_int32, vp, C.c_size_t]
     L.scmf_copy_in_pos.argtypes = [vp, vp, C.c_size_t]
     L.scmf_copy_in_omega.argtypes = [vp, vp, C.c_size_t]
     L.scmf_copy_in_density.argtypes = [vp, vp, C.c_size_t]
@@ -187,6 +188,13 @@ class System:
         out = np.empty(self.n_types * self.n_cells, np.float32)
         _check(lib().scmf_copy_out(self._h, OMEGA, out.ctypes.data, out.nbytes))
         return out
+
+    def copy_out_noalloc(self, what):
+        """scmf_copy_out_noalloc of positions (POS), counts (DENSITY) or omega (OMEGA)."""
+        n = 3 * self.n_beads if what == POS else self.n_types * self.n_cells
+        out = np.empty(n, np.float32 if what in (POS, OMEGA) else np.uint32)
+        _check(lib().scmf_copy_out_noalloc(self._h, what, out.ctypes.data, out.nbytes))
+        return out.reshape(-1, 3) if what == POS else out
 
     def set_positions(self, xyz):
         a = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
