@@ -267,6 +267,14 @@ int scmf_write_vtk_host(const char* vtk_path, const char* density_name, const in
  * leave it.  Call scmf_batch_sync before using a member's handle again. */
 typedef struct scmf_batch scmf_batch;
 int scmf_batch_create(scmf_batch** out, scmf_sys* const* members, int32_t n_members);
+/* Replace the members and give each a stop: from sweep count stop_sweeps[i]
+   on (its stream sweep word, scmf_stats().sweeps) member i does no more work
+   in the batch's steps, so members with different sweep targets share every
+   step; a negative stop (or stop_sweeps null) means none.  Takes effect for
+   the steps queued after it, without waiting for those queued before; members
+   new to the batch must have no work of their own pending. */
+int scmf_batch_update(scmf_batch* batch, scmf_sys* const* members, int32_t n_members,
+                      const int64_t* stop_sweeps);
 int scmf_batch_step(scmf_batch* batch, int32_t n_steps);
 int scmf_batch_sync(scmf_batch* batch);
 /* Force one sweep-kernel shape for every member (0 = each member's own). */
@@ -275,31 +283,55 @@ void scmf_batch_destroy(scmf_batch* batch);
 
 /* ---- serving: jobs submitted at any time, run by continuous batching -------
  * No counterpart in the reference.  A job is a system (description, options)
- * and a sweep count.  scmf_server_tick admits queued jobs in arrival order
- * while the device-memory budget and the slot count allow, runs every admitted
- * job through one batch (sweep, grid, omega per step, as scmf_step), and marks
- * the jobs that reached their sweep count DONE; their state stays resident
- * until released.  A finished job's result equals running it alone. */
+ * and a sweep count.  A worker thread admits queued jobs in arrival order
+ * while the device-memory budget and the slot count allow, creating and
+ * initialising each on its own stream while the running jobs keep stepping.
+ * scmf_server_tick brings admitted jobs into one batch, enqueues
+ * steps_per_tick steps of it (sweep, grid, omega per step, as scmf_step;
+ * each job stops on the device at its own sweep count, so a tick is never cut
+ * short for the job closest to its end) and returns without waiting for them
+ * while fewer than two ticks are queued; a job is DONE when the tick holding
+ * its last sweep has run, and its state stays resident until released.  A
+ * finished job's result equals running it alone. */
 #define SCMF_JOB_QUEUED 0
 #define SCMF_JOB_RUNNING 1
 #define SCMF_JOB_DONE 2
 #define SCMF_JOB_RELEASED 3
 #define SCMF_JOB_FAILED 4
+#define SCMF_JOB_CANCELLED 5
+
+/* Why a job was cancelled (scmf_job_info.reason). */
+#define SCMF_CANCEL_NONE 0
+#define SCMF_CANCEL_CALLER 1   /* scmf_server_cancel */
+#define SCMF_CANCEL_TIMEOUT 2  /* its wall-clock limit passed before it finished */
 
 typedef struct scmf_server_opts {
   uint64_t hbm_budget;    /* device bytes the jobs may hold (0: 200 GiB) */
-  int32_t max_jobs;       /* running jobs at most (0: 4096) */
-  int32_t steps_per_tick; /* sweeps per tick at most (0: 50) */
+  int32_t max_jobs;       /* admitted jobs at most (0: 4096) */
+  int32_t steps_per_tick; /* sweeps per tick (0: 50) */
+  double job_timeout_s;   /* default wall-clock limit of a job from its submission
+                             (0: none); a job not DONE by then is cancelled */
 } scmf_server_opts;
 
 typedef struct scmf_job_info {
   int32_t state;
-  int32_t pad;
-  int64_t sweeps_done, sweeps_target;
-  double t_submit, t_admit, t_done; /* seconds on the server clock */
+  int32_t reason;                   /* SCMF_CANCEL_* of a CANCELLED job */
+  int64_t sweeps_done, sweeps_target;  /* sweeps_done counts the sweeps enqueued */
+  double t_submit, t_admit, t_done; /* seconds on the server clock; t_done is when
+                                       the device finished the job's last step */
   uint64_t device_bytes;            /* admission estimate */
   int64_t beads;
 } scmf_job_info;
+
+typedef struct scmf_server_stats_t {
+  uint64_t ticks;        /* ticks enqueued */
+  uint64_t rebuilds;     /* changes of the batch's membership */
+  uint64_t admitted, finished, cancelled, failed;
+  double busy_s;         /* device time of the finished ticks */
+  double span_s;         /* from the start of the first tick to the end of the last
+                            finished one; busy_s / span_s is the GPU's busy fraction */
+  uint64_t used_bytes;   /* admission estimates of the jobs holding memory */
+} scmf_server_stats_t;
 
 typedef struct scmf_server scmf_server;
 int scmf_server_create(scmf_server** out, const scmf_server_opts* opts);
@@ -309,9 +341,19 @@ int scmf_server_tick(scmf_server* server, int32_t* n_queued, int32_t* n_running)
 int scmf_server_info(scmf_server* server, uint64_t job_id, scmf_job_info* out);
 /* Handle of a finished job, for scmf_copy_out / scmf_stats / the writers. */
 int scmf_server_job_system(scmf_server* server, uint64_t job_id, scmf_sys** out);
+/* Free a finished, failed or cancelled job's memory (once no queued step uses it). */
 int scmf_server_release(scmf_server* server, uint64_t job_id);
+/* Cancel a queued or running job: a queued one is never admitted; a running one
+   leaves the batch at the next tick and its memory is freed once the steps
+   queued with it have run.  Its state becomes CANCELLED (reason CALLER). */
+int scmf_server_cancel(scmf_server* server, uint64_t job_id);
+/* Wall-clock limit of one job from its submission (0: none), replacing the
+   server's default. */
+int scmf_server_set_timeout(scmf_server* server, uint64_t job_id, double seconds);
+int scmf_server_stats(scmf_server* server, scmf_server_stats_t* out);
 double scmf_server_clock(scmf_server* server);
 const char* scmf_server_last_error(void);
+/* Waits for the queued steps, then frees every job. */
 void scmf_server_destroy(scmf_server* server);
 
 const char* scmf_last_error(void);

@@ -98,14 +98,21 @@ class Summary(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_jobs", C.c_int32),
-                ("steps_per_tick", C.c_int32)]
+                ("steps_per_tick", C.c_int32), ("job_timeout_s", C.c_double)]
 
 
-JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_RELEASED, JOB_FAILED = range(5)
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_RELEASED, JOB_FAILED, JOB_CANCELLED = range(6)
+CANCEL_NONE, CANCEL_CALLER, CANCEL_TIMEOUT = range(3)
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("ticks", C.c_uint64), ("rebuilds", C.c_uint64), ("admitted", C.c_uint64),
+                ("finished", C.c_uint64), ("cancelled", C.c_uint64), ("failed", C.c_uint64),
+                ("busy_s", C.c_double), ("span_s", C.c_double), ("used_bytes", C.c_uint64)]
 
 
 class JobInfo(C.Structure):
-    _fields_ = [("state", C.c_int32), ("pad", C.c_int32), ("sweeps_done", C.c_int64),
+    _fields_ = [("state", C.c_int32), ("reason", C.c_int32), ("sweeps_done", C.c_int64),
                 ("sweeps_target", C.c_int64), ("t_submit", C.c_double), ("t_admit", C.c_double),
                 ("t_done", C.c_double), ("device_bytes", C.c_uint64), ("beads", C.c_int64)]
 

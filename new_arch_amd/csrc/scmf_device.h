@@ -78,7 +78,12 @@ struct KParams {
                                  // (x | y << 10 | z << 20), or null
   uint8_t* sched;                // per slot: the sweep's move schedule made by k_schedule
   int sched_stride;              // bytes per slot record
+  uint32_t stop;                 // batches: the member does no work once its sweep word
+                                 // reaches this (NO_STOP: never); ctr[1] says whether the
+                                 // current step's sweep ran
 };
+
+constexpr uint32_t NO_STOP = 0xFFFFFFFFu;
 
 // Accepted moves are counted in this many separate words, chosen by
 // workgroup, so the workgroups of a sweep do not queue on one address.
@@ -96,6 +101,20 @@ using KPC = const SCMF_CONST KParams;
 // The parameters of system `sys` seen through the constant address space.
 __device__ __forceinline__ KPC& param(const KParams* Ps, uint32_t sys) {
   return *(KPC*)(Ps + sys);
+}
+
+// A batch member whose sweep word has reached its stop sweeps no more.
+template <class KP>
+__device__ __forceinline__ bool sweep_stopped(const KP& kp) {
+  return ((const uint32_t*)kp.ctr)[0] >= kp.stop;
+}
+
+// The gridding and omega update of a step run only if the step's sweep ran
+// (k_advance sets ctr[1]); a member without a stop always runs them, also
+// outside the step sequence.
+template <class KP>
+__device__ __forceinline__ bool step_skipped(const KP& kp) {
+  return kp.stop != NO_STOP && ((const uint32_t*)kp.ctr)[1] == 0u;
 }
 
 __device__ __forceinline__ BlockRef block_ref(const BlockRef* bt) {

@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <deque>
@@ -30,6 +31,7 @@
 #include <vector>
 
 #include "../../include/scmf.h"
+#include "scmf_batch_tickets.h"
 #include "scmf_device.h"
 #include "scmf_host.h"
 #include "scmf_tiles.h"
@@ -292,12 +294,15 @@ __device__ __forceinline__ void wave_count(uint32_t acc, unsigned long long* acc
               (unsigned long long)acc);
 }
 
-// Advance the stream sweep word of systems [0, n) after their sweep.
+// Advance the stream sweep word of systems [0, n) after their sweep; a
+// member at its stop stays, and ctr[1] records whether the sweep ran.
 __global__ void __launch_bounds__(64) k_advance(const KParams* __restrict__ Ps, int n) {
   const int i = blockIdx.x * 64 + threadIdx.x;
   if (i < n) {
     uint32_t* c = Ps[i].ctr;
-    c[0] = c[0] + 1u;
+    const bool ran = c[0] < Ps[i].stop;
+    if (ran) c[0] = c[0] + 1u;
+    c[1] = ran ? 1u : 0u;
   }
 }
 
@@ -307,6 +312,7 @@ __global__ void __launch_bounds__(BLOCK) k_sweep_global(const KParams* __restric
                                                         const BlockRef* __restrict__ bt) {
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
+  if (sweep_stopped(kp)) return;
   SCMF_GLOBAL float* __restrict__ x = gptr((float*)kp.x);
   SCMF_GLOBAL float* __restrict__ y = gptr((float*)kp.y);
   SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
@@ -334,6 +340,7 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
   extern __shared__ float lds[];
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
+  if (sweep_stopped(kp)) return;
   SCMF_GLOBAL float* __restrict__ x = gptr((float*)kp.x);
   SCMF_GLOBAL float* __restrict__ y = gptr((float*)kp.y);
   SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
@@ -406,6 +413,7 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
   extern __shared__ float lds[];
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
+  if (sweep_stopped(kp)) return;
   SCMF_GLOBAL float* __restrict__ x = gptr((float*)kp.x);
   SCMF_GLOBAL float* __restrict__ y = gptr((float*)kp.y);
   SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
@@ -654,6 +662,7 @@ __global__ void __launch_bounds__(BLOCK) k_zero_counts(const KParams* __restrict
                                                        const BlockRef* __restrict__ bt) {
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
+  if (step_skipped(kp)) return;
   const int64_t stride = (int64_t)br.nblk * BLOCK;
   SCMF_GLOBAL uint32_t* __restrict__ c = gptr((uint32_t*)kp.counts);
   for (int64_t i = (int64_t)br.local * BLOCK + threadIdx.x; i < kp.count_words; i += stride) c[i] = 0u;
@@ -678,6 +687,7 @@ __global__ void __launch_bounds__(BLOCK) k_grid(const KParams* __restrict__ Ps,
                                                 const BlockRef* __restrict__ bt) {
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
+  if (step_skipped(kp)) return;
   const SCMF_GLOBAL float* __restrict__ x = gptr((const float*)kp.x);
   const SCMF_GLOBAL float* __restrict__ y = gptr((const float*)kp.y);
   const SCMF_GLOBAL float* __restrict__ z = gptr((const float*)kp.z);
@@ -727,6 +737,7 @@ __global__ void __launch_bounds__(BLOCK) k_omega(const KParams* __restrict__ Ps,
                                                  const BlockRef* __restrict__ bt, int cadence) {
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
+  if (step_skipped(kp)) return;
   if (cadence && !kp.omega_every_step && ((const uint32_t*)kp.ctr)[0] % 10u == 0u) return;
   const SCMF_GLOBAL uint32_t* __restrict__ words = gptr((const uint32_t*)kp.counts);
   SCMF_GLOBAL float* __restrict__ omega = gptr((float*)kp.omega);
@@ -1049,6 +1060,25 @@ int fail(int code, const std::string& msg) {
       return fail(SCMF_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
   } while (0)
 
+// The dynamic-LDS limit of a kernel is a process-wide attribute: it is only
+// ever raised (to the largest need of any system or batch so far), so a later
+// system with a smaller need never lowers it under one already running.
+hipError_t raise_lds_limit(const void* fn, size_t bytes) {
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, size_t>> set;
+  std::lock_guard<std::mutex> lock(mu);
+  for (auto& e : set)
+    if (e.first == fn) {
+      if (e.second >= bytes) return hipSuccess;
+      const hipError_t r = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+      if (r == hipSuccess) e.second = bytes;
+      return r;
+    }
+  const hipError_t r = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (r == hipSuccess) set.push_back({fn, bytes});
+  return r;
+}
+
 enum Kind { K_SWEEP = 0, K_GRID = 1, K_OMEGA = 2, K_KINDS = 3 };
 
 // Sweep kernel variants: L lanes per chain (0 = positions in global memory).
@@ -1247,6 +1277,7 @@ int derive(scmf_sys* s) {
     kp.A[t] = (float)A;
     kp.R[t] = (float)sqrt(A * 2);
   }
+  kp.stop = NO_STOP;
   kp.seed = s->opts.seed;
   kp.system_id = s->opts.system_id;
   kp.move_type = s->opts.move_type;
@@ -1372,7 +1403,7 @@ int choose_sweep(scmf_sys* s) {
     return fail(SCMF_E_ARG, "lanes_per_chain must be 1, 2, 4 or 8 and fit in LDS");
   const size_t need = best->bytes(s->nmax);
   if (need > 65536) {
-    hipError_t e = hipFuncSetAttribute(best->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need);
+    hipError_t e = raise_lds_limit(best->fn, need);
     if (e != hipSuccess) {
       (void)hipGetLastError();
       best = &kVariants[0];
@@ -1572,7 +1603,7 @@ int alloc_sort_buffers(scmf_sys* s) {
   if (s->tile_lds + MONO_LDS + 128 > 65536) {
     for (int v = 0; v < 16; ++v) {
       const void* f = grid_tiles_kernel(v & 1, v & 2, v & 4, v & 8);
-      if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->tile_lds) !=
+      if (raise_lds_limit(f, s->tile_lds) !=
           hipSuccess) {
         (void)hipGetLastError();
         return fail(SCMF_E_HIP, "cannot raise the LDS limit of the tiled gridding");
@@ -2805,50 +2836,123 @@ int scmf_density_to_vtk(const char* density_path, const char* vtk_path) {
 // (per sweep-kernel shape) over the workgroups of all members, each workgroup
 // looking up its member in a block table; the whole step is captured once in
 // a HIP graph and replayed.  Members keep their own memory and can still be
-// read or written through their handles between batch calls.
+// read or written through their handles between batch calls (after
+// scmf_batch_sync).  A member may carry a stop: from that sweep count on its
+// workgroups return at once, so members of different lengths share every
+// step.  A change of membership or stops builds the next generation of tables
+// and graph without waiting for the queued steps: the tables go to a
+// stream-ordered allocation filled from pinned staging on the batch stream,
+// the graph is captured on a stream of its own, and the previous generation's
+// memory is freed in stream order, its graph destroyed once an event after
+// its last launch has passed.
 struct scmf_batch {
   std::vector<scmf_sys*> m;
-  hipStream_t stream = nullptr;
-  KParams* d_Ps = nullptr;
+  std::vector<uint32_t> stop;    // per member: the sweep count it stops at (NO_STOP: none)
+  hipStream_t stream = nullptr;  // the steps
+  hipStream_t cap = nullptr;     // graph capture only
   struct Group {
     const SweepVariant* v;
     int nmax;
     size_t lds;
-    BlockRef* d_bt;
+    size_t bt;                   // byte offset of its block table in d_buf
     unsigned nblocks;
   };
+  // the current generation
+  uint8_t* d_buf = nullptr;      // KParams of every member, then the block tables
   std::vector<Group> groups;
-  BlockRef* d_bt_zero = nullptr;
-  BlockRef* d_bt_grid = nullptr;
-  BlockRef* d_bt_omega = nullptr;
+  size_t bt_zero = 0, bt_grid = 0, bt_omega = 0;
   unsigned n_zero = 0, n_grid = 0, n_omega = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int forced_L = 0;  // 0: each member's own shape
+  struct Retired {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    hipEvent_t done;
+  };
+  std::vector<Retired> retired;
+  struct Stage {
+    void* host = nullptr;
+    size_t cap = 0;
+    hipEvent_t used = nullptr;
+    bool pending = false;
+  };
+  Stage stage[3];                // pinned staging of the table uploads, used in turn
+  int next_stage = 0;
+  int forced_L = 0;              // 0: each member's own shape
   uint64_t steps = 0;
+  uint64_t generations = 0;
+  // tickets: events around enqueued steps, timed from an origin event
+  hipEvent_t origin = nullptr;
+  std::chrono::steady_clock::time_point origin_host;
+  struct Ticket {
+    uint64_t id;
+    hipEvent_t a, b;
+  };
+  std::deque<Ticket> tickets;
+  uint64_t next_ticket = 1;
 };
 
 namespace {
 
-int batch_free_tables(scmf_batch* b) {
-  for (auto& g : b->groups)
-    if (g.d_bt) (void)hipFree(g.d_bt);
-  b->groups.clear();
-  for (BlockRef** p : {&b->d_bt_zero, &b->d_bt_grid, &b->d_bt_omega})
-    if (*p) {
-      (void)hipFree(*p);
-      *p = nullptr;
+// Destroy the retired graphs whose last launch has run (all of them, waiting,
+// with `wait`).
+void batch_reclaim(scmf_batch* b, bool wait) {
+  std::vector<scmf_batch::Retired> keep;
+  for (auto& r : b->retired) {
+    const hipError_t q = wait ? hipEventSynchronize(r.done) : hipEventQuery(r.done);
+    if (q == hipSuccess) {
+      if (r.exec) (void)hipGraphExecDestroy(r.exec);
+      if (r.graph) (void)hipGraphDestroy(r.graph);
+      (void)hipEventDestroy(r.done);
+    } else {
+      (void)hipGetLastError();
+      keep.push_back(r);
     }
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
+  }
+  b->retired.swap(keep);
+}
+
+// Retire the current generation: its memory is freed in stream order after the
+// steps queued so far, its graph destroyed once they have run.
+int batch_retire(scmf_batch* b) {
+  if (b->exec || b->graph) {
+    hipEvent_t ev = nullptr;
+    HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(ev, b->stream));
+    b->retired.push_back({b->graph, b->exec, ev});
+    b->graph = nullptr;
+    b->exec = nullptr;
+  }
+  if (b->d_buf) {
+    HIP_TRY(hipFreeAsync(b->d_buf, b->stream));
+    b->d_buf = nullptr;
+  }
+  b->groups.clear();
   return SCMF_OK;
 }
 
-int upload_table(const std::vector<BlockRef>& t, BlockRef** d) {
-  HIP_TRY(hipMalloc(d, sizeof(BlockRef) * std::max<size_t>(1, t.size())));
-  if (!t.empty()) HIP_TRY(hipMemcpy(*d, t.data(), sizeof(BlockRef) * t.size(), hipMemcpyHostToDevice));
+// Copy host bytes to device memory on the batch stream through a pinned
+// staging slot, so the host does not wait for the steps queued before it.
+int batch_upload(scmf_batch* b, void* dst, const void* src, size_t bytes) {
+  scmf_batch::Stage& st = b->stage[b->next_stage];
+  b->next_stage = (b->next_stage + 1) % 3;
+  if (st.pending) {
+    HIP_TRY(hipEventSynchronize(st.used));
+    st.pending = false;
+  }
+  if (st.cap < bytes) {
+    const size_t cap = std::max(bytes, 2 * st.cap);
+    if (st.host) (void)hipHostFree(st.host);
+    st.host = nullptr;
+    st.cap = 0;
+    HIP_TRY(hipHostMalloc(&st.host, cap, hipHostMallocDefault));
+    st.cap = cap;
+  }
+  if (!st.used) HIP_TRY(hipEventCreateWithFlags(&st.used, hipEventDisableTiming));
+  memcpy(st.host, src, bytes);
+  HIP_TRY(hipMemcpyAsync(dst, st.host, bytes, hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipEventRecord(st.used, b->stream));
+  st.pending = true;
   return SCMF_OK;
 }
 
@@ -2858,17 +2962,32 @@ uint32_t stride_blocks(int64_t work) {
   return (uint32_t)std::max<int64_t>(1, std::min<int64_t>(nb, 1 << 16));
 }
 
-// Build the parameter array, block tables and the step graph.
+size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
+
+// Build the next generation: parameters, block tables and the step graph.
 int batch_build(scmf_batch* b) {
-  batch_free_tables(b);
+  batch_reclaim(b, false);
+  int rc = batch_retire(b);
+  if (rc) return rc;
   const int n = (int)b->m.size();
-  std::vector<KParams> ps(n);
-  for (int i = 0; i < n; ++i) {
-    ps[i] = b->m[i]->kp;
-    ps[i].cells = nullptr;  // the batch grids from positions
-    ps[i].sched = nullptr;  // and schedules inside the sweep
+  if (n == 0) return SCMF_OK;
+  // host image of the generation's device buffer
+  std::vector<uint8_t> blob(align16(sizeof(KParams) * n));
+  {
+    KParams* ps = (KParams*)blob.data();
+    for (int i = 0; i < n; ++i) {
+      ps[i] = b->m[i]->kp;
+      ps[i].cells = nullptr;  // the batch grids from positions
+      ps[i].sched = nullptr;  // and schedules inside the sweep
+      ps[i].stop = b->stop[i];
+    }
   }
-  HIP_TRY(hipMemcpy(b->d_Ps, ps.data(), sizeof(KParams) * n, hipMemcpyHostToDevice));
+  auto add_table = [&](const std::vector<BlockRef>& t) {
+    const size_t off = blob.size();
+    blob.resize(align16(off + sizeof(BlockRef) * std::max<size_t>(1, t.size())), 0);
+    if (!t.empty()) memcpy(blob.data() + off, t.data(), sizeof(BlockRef) * t.size());
+    return off;
+  };
   // sweep groups by kernel shape; members ordered by decreasing work first
   std::vector<int> order(n);
   for (int i = 0; i < n; ++i) order[i] = i;
@@ -2892,7 +3011,7 @@ int batch_build(scmf_batch* b) {
     for (size_t k = 0; k < b->groups.size(); ++k)
       if (b->groups[k].v == v) gi = (int)k;
     if (gi < 0) {
-      b->groups.push_back({v, 0, 0, nullptr, 0});
+      b->groups.push_back({v, 0, 0, 0, 0});
       tabs.emplace_back();
       gi = (int)b->groups.size() - 1;
     }
@@ -2905,15 +3024,12 @@ int batch_build(scmf_batch* b) {
   for (size_t k = 0; k < b->groups.size(); ++k) {
     scmf_batch::Group& g = b->groups[k];
     g.lds = g.v->bytes(g.nmax);
-    if (g.lds > 65536 &&
-        hipFuncSetAttribute(g.v->fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)g.lds) !=
-            hipSuccess) {
+    if (g.lds > 65536 && raise_lds_limit(g.v->fn, g.lds) != hipSuccess) {
       (void)hipGetLastError();
       return fail(SCMF_E_ARG, "batch members' chains are too long for one LDS tile shape");
     }
     g.nblocks = (unsigned)tabs[k].size();
-    int rc = upload_table(tabs[k], &g.d_bt);
-    if (rc) return rc;
+    g.bt = add_table(tabs[k]);
   }
   std::vector<BlockRef> tz, tg, to;
   for (int i : order) {
@@ -2927,76 +3043,182 @@ int batch_build(scmf_batch* b) {
     const uint32_t no = stride_blocks(s->n_cells);
     for (uint32_t k = 0; k < no; ++k) to.push_back({(uint32_t)i, k, no, 0});
   }
-  int rc;
-  if ((rc = upload_table(tz, &b->d_bt_zero)) || (rc = upload_table(tg, &b->d_bt_grid)) ||
-      (rc = upload_table(to, &b->d_bt_omega)))
-    return rc;
+  b->bt_zero = add_table(tz);
+  b->bt_grid = add_table(tg);
+  b->bt_omega = add_table(to);
   b->n_zero = (unsigned)tz.size();
   b->n_grid = (unsigned)tg.size();
   b->n_omega = (unsigned)to.size();
-  // capture one step
-  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  const KParams* Ps = b->d_Ps;
+  HIP_TRY(hipMallocAsync((void**)&b->d_buf, blob.size(), b->stream));
+  if ((rc = batch_upload(b, b->d_buf, blob.data(), blob.size()))) return rc;
+  // capture one step on the capture stream
+  const KParams* Ps = (const KParams*)b->d_buf;
+  auto table = [&](size_t off) { return (const BlockRef*)(b->d_buf + off); };
+  // relaxed: other host threads (the job server's worker) may allocate meanwhile
+  HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
   for (const scmf_batch::Group& g : b->groups) {
     if (g.v->L == 0) {
-      hipLaunchKernelGGL(k_sweep_global, dim3(g.nblocks), dim3(BLOCK), 0, b->stream, Ps,
-                         (const BlockRef*)g.d_bt);
+      hipLaunchKernelGGL(k_sweep_global, dim3(g.nblocks), dim3(BLOCK), 0, b->cap, Ps, table(g.bt));
     } else {
-      const BlockRef* bt = g.d_bt;
+      const BlockRef* bt = table(g.bt);
       int nmax = g.nmax;
       void* args[] = {(void*)&Ps, (void*)&bt, (void*)&nmax};
-      (void)hipLaunchKernel(g.v->fn, dim3(g.nblocks), dim3(64), args, g.lds, b->stream);
+      (void)hipLaunchKernel(g.v->fn, dim3(g.nblocks), dim3(64), args, g.lds, b->cap);
     }
   }
-  hipLaunchKernelGGL(k_advance, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, b->stream, Ps, n);
-  hipLaunchKernelGGL(k_zero_counts, dim3(b->n_zero), dim3(BLOCK), 0, b->stream, Ps,
-                     (const BlockRef*)b->d_bt_zero);
+  hipLaunchKernelGGL(k_advance, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, b->cap, Ps, n);
+  hipLaunchKernelGGL(k_zero_counts, dim3(b->n_zero), dim3(BLOCK), 0, b->cap, Ps, table(b->bt_zero));
   if (b->n_grid)
-    hipLaunchKernelGGL(k_grid, dim3(b->n_grid), dim3(BLOCK), 0, b->stream, Ps,
-                       (const BlockRef*)b->d_bt_grid);
-  for (int i = 0; i < n; ++i)
-    if (b->m[i]->tile_lds && b->m[i]->sorted)
-      (void)launch_tiled_grid(b->m[i], b->stream, Ps, i, false);
-  hipLaunchKernelGGL(k_omega, dim3(b->n_omega), dim3(BLOCK), 0, b->stream, Ps,
-                     (const BlockRef*)b->d_bt_omega, 1);
-  hipError_t e = hipStreamEndCapture(b->stream, &b->graph);
+    hipLaunchKernelGGL(k_grid, dim3(b->n_grid), dim3(BLOCK), 0, b->cap, Ps, table(b->bt_grid));
+  int trc = SCMF_OK;
+  for (int i = 0; i < n && trc == SCMF_OK; ++i)
+    if (b->m[i]->tile_lds && b->m[i]->sorted) trc = launch_tiled_grid(b->m[i], b->cap, Ps, i, false);
+  hipLaunchKernelGGL(k_omega, dim3(b->n_omega), dim3(BLOCK), 0, b->cap, Ps, table(b->bt_omega), 1);
+  hipError_t e = hipStreamEndCapture(b->cap, &b->graph);
+  if (trc) return trc;
   if (e != hipSuccess) return fail(SCMF_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
   HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->generations += 1;
+  return SCMF_OK;
+}
+
+// Members must be idle on their own streams and, if gridded in tiles, ordered.
+int batch_prepare_members(scmf_sys* const* members, int32_t n) {
+  for (int32_t i = 0; i < n; ++i) {
+    scmf_sys* s = members[i];
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    if (s->tile_lds && !s->sorted && s->sort_every > 0) {
+      int rc = sort_chains(s);
+      if (rc) return rc;
+      HIP_TRY(hipStreamSynchronize(s->stream));
+    }
+  }
+  return SCMF_OK;
+}
+
+int batch_check_members(scmf_sys* const* members, int32_t n) {
+  if (n < 0 || (n > 0 && !members)) return fail(SCMF_E_ARG, "batch needs members");
+  for (int32_t i = 0; i < n; ++i) {
+    if (!members[i]) return fail(SCMF_E_ARG, "null batch member");
+    for (int32_t j = 0; j < i; ++j)
+      if (members[j] == members[i]) return fail(SCMF_E_ARG, "a system appears twice in the batch");
+  }
+  return SCMF_OK;
+}
+
+uint32_t stop_of(int64_t v) {
+  return v < 0 || v >= (int64_t)NO_STOP ? NO_STOP : (uint32_t)v;
+}
+
+// The launches of n steps on the batch stream (host bookkeeping included).
+int batch_launch(scmf_batch* b, int32_t n_steps) {
+  batch_reclaim(b, false);
+  for (int32_t k = 0; k < n_steps; ++k) {
+    // spatial reordering of tile-gridded members, between graph replays
+    bool reordered = false;
+    for (size_t i = 0; i < b->m.size(); ++i) {
+      scmf_sys* s = b->m[i];
+      if (s->sweeps >= b->stop[i]) continue;
+      if (s->tile_lds && s->sort_every > 0 && s->sweeps % (uint64_t)s->sort_every == 0 &&
+          s->sweeps > 0) {
+        HIP_TRY(hipStreamSynchronize(b->stream));
+        int rc = sort_chains(s);
+        if (rc) return rc;
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        reordered = true;
+      }
+    }
+    if (reordered) {
+      int rc = batch_build(b);
+      if (rc) return rc;
+    }
+    if (!b->exec) break;  // no members
+    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    for (size_t i = 0; i < b->m.size(); ++i) {
+      scmf_sys* s = b->m[i];
+      s->cells_fresh = false;
+      if (s->sweeps < b->stop[i]) {
+        s->sweeps += 1;
+        s->moves += (uint64_t)s->n_beads;
+      }
+    }
+    b->steps += 1;
+  }
   return SCMF_OK;
 }
 
 }  // namespace
 
+// Steps of a batch behind a ticket, for the job server: events around them on
+// the batch stream, read without waiting.
+namespace scmf_batch_tickets {
+
+int enqueue(scmf_batch* b, int32_t n_steps, uint64_t* ticket) {
+  scmf_batch::Ticket t{b->next_ticket, nullptr, nullptr};
+  HIP_TRY(hipEventCreate(&t.a));
+  HIP_TRY(hipEventCreate(&t.b));
+  HIP_TRY(hipEventRecord(t.a, b->stream));
+  int rc = batch_launch(b, n_steps);
+  if (rc) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+    return rc;
+  }
+  HIP_TRY(hipEventRecord(t.b, b->stream));
+  b->next_ticket += 1;
+  b->tickets.push_back(t);
+  *ticket = t.id;
+  return SCMF_OK;
+}
+
+// *done = 1 once the steps of `ticket` have run (wait: block until then);
+// *t_end = when they ended, in seconds after the batch's origin event;
+// *busy = their device time.  A finished ticket is forgotten after this.
+int poll(scmf_batch* b, uint64_t ticket, bool wait, int* done, double* t_end, double* busy) {
+  *done = 0;
+  for (auto it = b->tickets.begin(); it != b->tickets.end(); ++it) {
+    if (it->id != ticket) continue;
+    const hipError_t q = wait ? hipEventSynchronize(it->b) : hipEventQuery(it->b);
+    if (q == hipErrorNotReady) return SCMF_OK;
+    HIP_TRY(q);
+    float ms_end = 0, ms_busy = 0;
+    HIP_TRY(hipEventElapsedTime(&ms_end, b->origin, it->b));
+    HIP_TRY(hipEventElapsedTime(&ms_busy, it->a, it->b));
+    *t_end = ms_end * 1e-3;
+    *busy = ms_busy * 1e-3;
+    (void)hipEventDestroy(it->a);
+    (void)hipEventDestroy(it->b);
+    b->tickets.erase(it);
+    *done = 1;
+    return SCMF_OK;
+  }
+  return fail(SCMF_E_ARG, "unknown ticket");
+}
+
+std::chrono::steady_clock::time_point origin(scmf_batch* b) { return b->origin_host; }
+
+}  // namespace scmf_batch_tickets
+
 extern "C" {
 
 int scmf_batch_create(scmf_batch** out, scmf_sys* const* members, int32_t n_members) {
-  if (!out || !members || n_members <= 0) return fail(SCMF_E_ARG, "batch needs members");
+  if (!out) return fail(SCMF_E_ARG, "out is null");
   *out = nullptr;
+  if (n_members <= 0) return fail(SCMF_E_ARG, "batch needs members");
+  int rc = batch_check_members(members, n_members);
+  if (rc) return rc;
   scmf_batch* b = new scmf_batch();
-  for (int32_t i = 0; i < n_members; ++i) {
-    if (!members[i]) {
-      delete b;
-      return fail(SCMF_E_ARG, "null batch member");
-    }
-    for (scmf_sys* o : b->m)
-      if (o == members[i]) {
-        delete b;
-        return fail(SCMF_E_ARG, "a system appears twice in the batch");
-      }
-    b->m.push_back(members[i]);
-  }
-  int rc = SCMF_OK;
+  b->m.assign(members, members + n_members);
+  b->stop.assign(n_members, NO_STOP);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&b->d_Ps, sizeof(KParams) * n_members) != hipSuccess) {
+      hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->origin) != hipSuccess || hipEventRecord(b->origin, b->stream) != hipSuccess ||
+      hipEventSynchronize(b->origin) != hipSuccess) {
     (void)hipGetLastError();
-    rc = fail(SCMF_E_HIP, "batch allocation failed");
+    rc = fail(SCMF_E_HIP, "batch stream set-up failed");
   }
-  for (scmf_sys* s : b->m) {
-    if (rc) break;
-    if (hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(SCMF_E_HIP, "member sync failed");
-    if (!rc && s->tile_lds && !s->sorted && s->sort_every > 0) rc = sort_chains(s);
-    if (!rc && hipStreamSynchronize(s->stream) != hipSuccess) rc = fail(SCMF_E_HIP, "member sync failed");
-  }
+  b->origin_host = std::chrono::steady_clock::now();
+  if (!rc) rc = batch_prepare_members(members, n_members);
   if (!rc) rc = batch_build(b);
   if (rc) {
     scmf_batch_destroy(b);
@@ -3006,11 +3228,39 @@ int scmf_batch_create(scmf_batch** out, scmf_sys* const* members, int32_t n_memb
   return SCMF_OK;
 }
 
+int scmf_batch_update(scmf_batch* b, scmf_sys* const* members, int32_t n_members,
+                      const int64_t* stop_sweeps) {
+  if (!b) return fail(SCMF_E_ARG, "null batch");
+  int rc = batch_check_members(members, n_members);
+  if (rc) return rc;
+  // members that are new to the batch must be idle on their own streams
+  std::vector<scmf_sys*> fresh;
+  for (int32_t i = 0; i < n_members; ++i)
+    if (std::find(b->m.begin(), b->m.end(), members[i]) == b->m.end()) fresh.push_back(members[i]);
+  if ((rc = batch_prepare_members(fresh.data(), (int32_t)fresh.size()))) return rc;
+  b->m.assign(members, members + n_members);
+  b->stop.assign(n_members, NO_STOP);
+  if (stop_sweeps)
+    for (int32_t i = 0; i < n_members; ++i) b->stop[i] = stop_of(stop_sweeps[i]);
+  return batch_build(b);
+}
+
 void scmf_batch_destroy(scmf_batch* b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  batch_free_tables(b);
-  if (b->d_Ps) (void)hipFree(b->d_Ps);
+  (void)batch_retire(b);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_reclaim(b, true);
+  for (auto& t : b->tickets) {
+    (void)hipEventDestroy(t.a);
+    (void)hipEventDestroy(t.b);
+  }
+  for (auto& st : b->stage) {
+    if (st.host) (void)hipHostFree(st.host);
+    if (st.used) (void)hipEventDestroy(st.used);
+  }
+  if (b->origin) (void)hipEventDestroy(b->origin);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -3018,50 +3268,18 @@ void scmf_batch_destroy(scmf_batch* b) {
 int scmf_batch_step(scmf_batch* b, int32_t n_steps) {
   if (!b) return fail(SCMF_E_ARG, "null batch");
   if (n_steps < 0) return fail(SCMF_E_ARG, "n_steps must be non-negative");
-  for (int32_t k = 0; k < n_steps; ++k) {
-    // spatial reordering of tile-gridded members, between graph replays
-    bool rebuilt = false;
-    for (scmf_sys* s : b->m) {
-      if (s->tile_lds && s->sort_every > 0 && s->sweeps % (uint64_t)s->sort_every == 0 &&
-          s->sweeps > 0) {
-        HIP_TRY(hipStreamSynchronize(b->stream));
-        int rc = sort_chains(s);
-        if (rc) return rc;
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        rebuilt = true;
-      }
-    }
-    if (rebuilt) {
-      std::vector<KParams> ps(b->m.size());
-      for (size_t i = 0; i < b->m.size(); ++i) {
-        ps[i] = b->m[i]->kp;
-        ps[i].cells = nullptr;
-        ps[i].sched = nullptr;
-      }
-      HIP_TRY(hipMemcpyAsync(b->d_Ps, ps.data(), sizeof(KParams) * ps.size(),
-                             hipMemcpyHostToDevice, b->stream));
-      HIP_TRY(hipStreamSynchronize(b->stream));
-    }
-    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
-    for (scmf_sys* s : b->m) {
-      s->cells_fresh = false;
-      s->sweeps += 1;
-      s->moves += (uint64_t)s->n_beads;
-    }
-    b->steps += 1;
-  }
-  return SCMF_OK;
+  return batch_launch(b, n_steps);
 }
 
 int scmf_batch_sync(scmf_batch* b) {
   if (!b) return fail(SCMF_E_ARG, "null batch");
   HIP_TRY(hipStreamSynchronize(b->stream));
+  batch_reclaim(b, false);
   return SCMF_OK;
 }
 
 int scmf_batch_set_lanes(scmf_batch* b, int32_t lanes_per_chain) {
   if (!b) return fail(SCMF_E_ARG, "null batch");
-  HIP_TRY(hipStreamSynchronize(b->stream));
   b->forced_L = lanes_per_chain;
   return batch_build(b);
 }

@@ -309,6 +309,7 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   __shared__ uint8_t mono_s[MONO_LDS + 4];
   __shared__ int reg_s[27];
   KPC& kp = param(Ps, (uint32_t)sys);
+  if (step_skipped(kp)) return;
   if (threadIdx.x < 27) reg_s[threadIdx.x] = g.reg_off[threadIdx.x];
   // per-axis tables after the window: cell -> offset in the window (scaled
   // by the strides of the window box; with PAIR in bytes), or OUT_OF_WINDOW
@@ -632,6 +633,7 @@ __global__ void __launch_bounds__(SUM_THREADS) k_tiles_sum(const KParams* __rest
                                                    TileGeo g,
                                                    const uint16_t* __restrict__ images) {
   KPC& kp = param(Ps, (uint32_t)sys);
+  if (step_skipped(kp)) return;
   SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
   SCMF_GLOBAL uint16_t* __restrict__ counts16 = gptr((uint16_t*)kp.counts);
   uint32_t* __restrict__ overflow = (uint32_t*)kp.overflow;
@@ -708,6 +710,7 @@ __global__ void __launch_bounds__(SUM_THREADS) k_tiles_sum_pair(const KParams* _
                                                                 int sys, TileGeo g,
                                                                 const uint32_t* __restrict__ img) {
   KPC& kp = param(Ps, (uint32_t)sys);
+  if (step_skipped(kp)) return;
   SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
   SCMF_GLOBAL uint16_t* __restrict__ counts16 = gptr((uint16_t*)kp.counts);
   uint32_t* __restrict__ overflow = (uint32_t*)kp.overflow;

@@ -8,8 +8,8 @@ import os
 
 import numpy as np
 
-from .abi import (DENSITY, OMEGA, POS, SCMF_OK, Desc, JobInfo, Opts, ServerOpts, Stats,
-                  Summary, SystemSpec, Timing, make_opts)
+from .abi import (DENSITY, OMEGA, POS, SCMF_OK, Desc, JobInfo, Opts, ServerOpts, ServerStats,
+                  Stats, Summary, SystemSpec, Timing, make_opts)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libscmf.so")
@@ -71,6 +71,7 @@ def lib():
     L.scmf_batch_step.argtypes = [vp, C.c_int32]
     L.scmf_batch_sync.argtypes = [vp]
     L.scmf_batch_set_lanes.argtypes = [vp, C.c_int32]
+    L.scmf_batch_update.argtypes = [vp, C.POINTER(vp), C.c_int32, vp]
     L.scmf_batch_destroy.argtypes = [vp]
     L.scmf_batch_destroy.restype = None
     L.scmf_server_create.argtypes = [C.POINTER(vp), C.POINTER(ServerOpts)]
@@ -80,6 +81,9 @@ def lib():
     L.scmf_server_info.argtypes = [vp, C.c_uint64, C.POINTER(JobInfo)]
     L.scmf_server_job_system.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
     L.scmf_server_release.argtypes = [vp, C.c_uint64]
+    L.scmf_server_cancel.argtypes = [vp, C.c_uint64]
+    L.scmf_server_set_timeout.argtypes = [vp, C.c_uint64, C.c_double]
+    L.scmf_server_stats.argtypes = [vp, C.POINTER(ServerStats)]
     L.scmf_server_clock.argtypes = [vp]
     L.scmf_server_clock.restype = C.c_double
     L.scmf_server_last_error.restype = C.c_char_p
@@ -300,6 +304,16 @@ class Batch:
     def set_lanes(self, lanes_per_chain):
         _check(lib().scmf_batch_set_lanes(self._h, int(lanes_per_chain)))
 
+    def update(self, members, stops=None):
+        """New members, each stopping at its sweep count in `stops` (None or -1: never)."""
+        self.members = list(members)
+        arr = (C.c_void_p * max(1, len(self.members)))(*[m._h.value for m in self.members])
+        st = None
+        if stops is not None:
+            st = np.ascontiguousarray([-1 if v is None else int(v) for v in stops], np.int64)
+        _check(lib().scmf_batch_update(self._h, arr, len(self.members),
+                                       st.ctypes.data if st is not None else None))
+
     def close(self):
         if self._h:
             lib().scmf_batch_destroy(self._h)
@@ -341,8 +355,8 @@ class JobView(System):
 class Server:
     """Continuous-batching job server (see include/scmf.h)."""
 
-    def __init__(self, hbm_budget=0, max_jobs=0, steps_per_tick=0):
-        o = ServerOpts(int(hbm_budget), int(max_jobs), int(steps_per_tick))
+    def __init__(self, hbm_budget=0, max_jobs=0, steps_per_tick=0, job_timeout_s=0.0):
+        o = ServerOpts(int(hbm_budget), int(max_jobs), int(steps_per_tick), float(job_timeout_s))
         self._h = C.c_void_p()
         _scheck(lib().scmf_server_create(C.byref(self._h), C.byref(o)))
         self.specs = {}
@@ -372,6 +386,17 @@ class Server:
 
     def release(self, jid):
         _scheck(lib().scmf_server_release(self._h, int(jid)))
+
+    def cancel(self, jid):
+        _scheck(lib().scmf_server_cancel(self._h, int(jid)))
+
+    def set_timeout(self, jid, seconds):
+        _scheck(lib().scmf_server_set_timeout(self._h, int(jid), float(seconds)))
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _scheck(lib().scmf_server_stats(self._h, C.byref(st)))
+        return st
 
     def clock(self) -> float:
         return lib().scmf_server_clock(self._h)

@@ -79,3 +79,35 @@ def test_batch_rejects_duplicates():
     with pytest.raises(scmf.ScmfError):
         scmf.Batch([g, g])
     g.close()
+
+
+def test_stops_and_membership_changes_while_steps_are_queued():
+    """Members stop on the device at their own sweep counts (the batch keeps
+    stepping past them), and membership changes take effect for the steps
+    queued after them without waiting for those queued before: every member
+    ends where its own number of scmf_step calls alone leaves it."""
+    banner("stops and membership changes")
+    systems = W.batched_mix(n_systems=6, seed=9, sizes=(16, 24), lengths=(16, 24, 32))
+    mem = make(systems)
+    stops = [5, 9, 14, 3, 11, 20]
+    b = scmf.Batch(mem[:4])
+    b.update(mem[:4], stops[:4])
+    b.step(7)                                 # members 0 and 3 stop inside these steps
+    b.update([mem[1], mem[2], mem[4]], [stops[1], stops[2], stops[4]])   # 0, 3 leave; 4 joins
+    b.step(6)
+    b.update([mem[2], mem[4], mem[5]], [stops[2], stops[4], stops[5]])   # 5 joins
+    b.step(25)
+    b.sync()
+    alone = make(systems)
+    for i, g in enumerate(alone):
+        for k in range(1, stops[i] + 1):
+            g.step(k)
+    for i, (a, m) in enumerate(zip(alone, mem)):
+        assert m.stats().sweeps == stops[i], (i, m.stats().sweeps)
+        assert same(a.positions(), m.positions()), f"member {i} positions"
+        np.testing.assert_array_equal(a.density(), m.density(), err_msg=f"member {i} density")
+        assert same(a.omega(), m.omega()), f"member {i} omega"
+        assert a.stats().accepts == m.stats().accepts
+    b.close()
+    for g in alone + mem:
+        g.close()

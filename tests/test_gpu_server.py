@@ -1,12 +1,15 @@
 """Serving: jobs of mixed size and length submitted over time, admitted
 against a memory budget, run by continuous batching; every finished job equals
 the same job run alone (bit for bit), and memory bookkeeping holds."""
+import time
+
 import numpy as np
 import pytest
 
 from new_arch_amd import scmf
 from new_arch_amd import workloads as W
-from new_arch_amd.abi import JOB_DONE, JOB_QUEUED, JOB_RELEASED, make_opts
+from new_arch_amd.abi import (CANCEL_CALLER, CANCEL_TIMEOUT, JOB_CANCELLED, JOB_DONE, JOB_QUEUED,
+                              JOB_RELEASED, make_opts)
 
 pytestmark = pytest.mark.gpu
 
@@ -79,11 +82,18 @@ def test_admission_against_budget():
     need = max(est(spec) for spec, _, _ in js)
     sv = scmf.Server(hbm_budget=int(need * 2.5), steps_per_tick=5)
     ids = [sv.submit(spec, opts, sweeps) for spec, opts, sweeps in js]
-    q, r = sv.tick()
-    assert r >= 1 and q + r <= len(js)
-    seen_queued = q > 0
-    for _ in range(300):
+    seen_queued = False
+    for _ in range(3000):          # admission runs on the server's worker thread
         q, r = sv.tick()
+        assert q + r <= len(js)
+        seen_queued |= q > 0 and r >= 1
+        if r >= 1:
+            break
+        time.sleep(0.001)
+    assert r >= 1
+    for _ in range(3000):
+        q, r = sv.tick()
+        seen_queued |= q > 0 and r >= 1
         for jid in ids:                   # finished jobs give their memory back
             if sv.info(jid).state == JOB_DONE:
                 sv.release(jid)
@@ -96,4 +106,73 @@ def test_admission_against_budget():
     with pytest.raises(scmf.ScmfError):
         tiny.submit(spec, opts, 5)
     tiny.close()
+    sv.close()
+
+
+def test_cancel_and_timeout_leave_the_others_unchanged():
+    """A running job cancelled mid-run, a job that passes its wall-clock limit
+    and a queued job cancelled before admission all end CANCELLED with their
+    memory returned; every other job still equals its run alone."""
+    banner("cancel and timeout")
+    js = jobs(6, seed=21)
+    long_spec, _ = W.served_job(np.random.default_rng(4), n_range=(24, 24), grid_range=(16, 16))
+    sv = scmf.Server(steps_per_tick=5)
+    ids = [sv.submit(spec, opts, sweeps) for spec, opts, sweeps in js]
+    victim = sv.submit(long_spec, make_opts(seed=900, system_id=90, density_bits=16), 100_000)
+    late = sv.submit(long_spec, make_opts(seed=901, system_id=91, density_bits=16), 100_000)
+    sv.set_timeout(late, 0.5)
+    for _ in range(200):
+        sv.tick()
+        if sv.info(victim).state == 1 and sv.info(victim).sweeps_done >= 10:
+            break
+    assert sv.info(victim).sweeps_done >= 10
+    sv.cancel(victim)
+    for _ in range(5000):
+        q, r = sv.tick()
+        if q == 0 and r == 0:
+            break
+    assert sv.info(victim).state == JOB_CANCELLED and sv.info(victim).reason == CANCEL_CALLER
+    assert sv.info(late).state == JOB_CANCELLED and sv.info(late).reason == CANCEL_TIMEOUT
+    for jid, (spec, opts, sweeps) in zip(ids, js):
+        assert sv.info(jid).state == JOB_DONE
+        got = sv.job(jid)
+        ref = alone(spec, opts, sweeps)
+        assert same(got.positions(), ref.positions()), f"job {jid}"
+        np.testing.assert_array_equal(got.density(), ref.density())
+        assert same(got.omega(), ref.omega())
+        ref.close()
+        sv.release(jid)
+    # a queued job cancelled before admission is never admitted: the budget
+    # holds the first two jobs, so the third waits at the head of the queue
+    # and the fourth behind it
+    b = [sv.info(j).device_bytes for j in ids[:3]]
+    tight = scmf.Server(hbm_budget=b[0] + b[1] + b[2] // 2, steps_per_tick=5)
+    first = [tight.submit(spec, opts, 200) for spec, opts, _ in js[:3]]
+    waiting = tight.submit(js[3][0], js[3][1], 5)
+    tight.tick()
+    tight.cancel(waiting)
+    assert tight.info(waiting).state == JOB_CANCELLED
+    done = set()
+    for _ in range(5000):
+        q, r = tight.tick()
+        for j in first:                   # finished jobs give their memory back
+            if j not in done and tight.info(j).state == JOB_DONE:
+                done.add(j)
+                tight.release(j)
+        if q == 0 and r == 0:
+            break
+    assert done == set(first)
+    assert tight.info(waiting).t_admit == 0.0
+    assert tight.stats().cancelled == 1
+    st = sv.stats()
+    assert st.cancelled == 2 and st.finished == len(js)
+    for _ in range(500):              # the worker frees released memory
+        if sv.stats().used_bytes == 0 and tight.stats().used_bytes == 0:
+            break
+        sv.tick()
+        tight.tick()
+        time.sleep(0.01)
+    assert sv.stats().used_bytes == 0 and tight.stats().used_bytes == 0
+    assert 0 < st.busy_s <= st.span_s * 1.001
+    tight.close()
     sv.close()
