@@ -254,12 +254,35 @@ def batched_vs_plain(steps, warmup, n_systems=64):
             "batched_ms_per_step": t_batch * 1e3 / steps}
 
 
-def served(n_jobs, rate, seed=0, steps_per_tick=25):
-    """Poisson stream of jobs into the continuous-batching server, driven in
-    wall-clock time: p50/p99 latency (finish - arrival) and aggregate
-    bead-moves/s over the run."""
-    import numpy as np
+def plain_jobs(jobs, seed_base=10_000):
+    """The same jobs run one at a time with the plain per-system kernels
+    (create, grid, omega, then scmf_step for every sweep): bead-moves/s."""
+    from new_arch_amd import scmf
+    from new_arch_amd.abi import make_opts
 
+    moves = 0
+    t0 = time.perf_counter()
+    for k, (spec, sweeps) in enumerate(jobs):
+        g = scmf.System(spec, make_opts(seed=seed_base + k, system_id=k, omega_every_step=True,
+                                        density_bits=16))
+        g.grid_density()
+        g.update_omega()
+        for step in range(1, sweeps + 1):
+            g.step(step)
+        g.sync()
+        g.close()
+        moves += spec.n_beads * sweeps
+    dt = time.perf_counter() - t0
+    return {"jobs": len(jobs), "moves": int(moves), "seconds": round(dt, 2),
+            "bead_moves_per_s": moves / dt}
+
+
+def served(n_jobs, rate, seed=0, steps_per_tick=25, plain_sample=24):
+    """Poisson stream of jobs into the continuous-batching server, driven in
+    wall-clock time: p50/p99 latency (finish - arrival), aggregate
+    bead-moves/s over the run and the GPU's busy fraction (device time of the
+    server's ticks over their span); then the first `plain_sample` jobs of the
+    same mix run one at a time with the plain kernels."""
     from new_arch_amd import scmf
     from new_arch_amd import workloads as W
     from new_arch_amd.abi import JOB_DONE, make_opts
@@ -294,16 +317,25 @@ def served(n_jobs, rate, seed=0, steps_per_tick=25):
             sv.release(jid)
         if q == 0 and r == 0 and submitted < n_jobs:
             time.sleep(max(0.0, min(0.01, arrivals[submitted] - (sv.clock() - t0))))
+    st = sv.stats()
     sv.close()
     lat = np.array(lat)
     agg = moves / (t_last - t_first)
+    busy = st.busy_s / st.span_s if st.span_s > 0 else None
     log(f"served: {n_jobs} jobs at {rate}/s; p50 {np.percentile(lat, 50):.3f} s, "
-        f"p99 {np.percentile(lat, 99):.3f} s, {agg:.3e} bead-moves/s")
+        f"p99 {np.percentile(lat, 99):.3f} s, {agg:.3e} bead-moves/s, GPU busy {busy}")
+    plain = plain_jobs(js[:plain_sample]) if plain_sample else None
+    if plain:
+        log(f"served mix one at a time (first {plain_sample} jobs): {plain}")
     return {"workload": "served_jobs", "jobs": n_jobs, "arrival_rate_per_s": rate,
             "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()), "aggregate_bead_moves_per_s": agg,
             "moves": int(moves), "span_s": t_last - t_first,
+            "gpu_busy_fraction": busy, "ticks": int(st.ticks), "rebuilds": int(st.rebuilds),
+            "steps_per_tick": steps_per_tick,
+            "plain_one_at_a_time": plain,
+            "served_over_plain": agg / plain["bead_moves_per_s"] if plain else None,
             "job_mix": "AB diblock or ABA triblock melts, N 16-128, grid 16^3-128^3 (Re/8 "
                        "cells), chiN 10-50, 500-5000 sweeps, 128 beads/Re^3, uint16"}
 
@@ -583,8 +615,8 @@ def main():
     ap.add_argument("--no-largest", action="store_true")
     ap.add_argument("--no-small", action="store_true")
     ap.add_argument("--small-steps", type=int, default=10000)  # the small melt's whole run
-    ap.add_argument("--served-jobs", type=int, default=200)
-    ap.add_argument("--served-rate", type=float, default=6.0)
+    ap.add_argument("--served-jobs", type=int, default=300)
+    ap.add_argument("--served-rate", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this engine runs one system per GPU and measures a single GPU; "

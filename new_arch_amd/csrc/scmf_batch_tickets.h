@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <chrono>
+#include <mutex>
 
 #include "../../include/scmf.h"
 
@@ -18,5 +19,10 @@ int enqueue(scmf_batch* b, int32_t n_steps, uint64_t* ticket);
 int poll(scmf_batch* b, uint64_t ticket, bool wait, int* done, double* t_end, double* busy);
 // Host time of the batch's origin event.
 std::chrono::steady_clock::time_point origin(scmf_batch* b);
+// Held around every graph capture of a batch: a thread that creates or
+// destroys systems while another may be capturing (the job server's worker)
+// takes it too, since the runtime abandons a capture when another thread
+// synchronises or frees meanwhile.
+std::mutex& capture_mutex();
 
 }  // namespace scmf_batch_tickets

@@ -3054,7 +3054,7 @@ int batch_build(scmf_batch* b) {
   // capture one step on the capture stream
   const KParams* Ps = (const KParams*)b->d_buf;
   auto table = [&](size_t off) { return (const BlockRef*)(b->d_buf + off); };
-  // relaxed: other host threads (the job server's worker) may allocate meanwhile
+  std::lock_guard<std::mutex> capture_lock(scmf_batch_tickets::capture_mutex());
   HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
   for (const scmf_batch::Group& g : b->groups) {
     if (g.v->L == 0) {
@@ -3196,6 +3196,11 @@ int poll(scmf_batch* b, uint64_t ticket, bool wait, int* done, double* t_end, do
 }
 
 std::chrono::steady_clock::time_point origin(scmf_batch* b) { return b->origin_host; }
+
+std::mutex& capture_mutex() {
+  static std::mutex mu;
+  return mu;
+}
 
 }  // namespace scmf_batch_tickets
 

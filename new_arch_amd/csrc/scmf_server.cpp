@@ -159,7 +159,10 @@ void worker_loop(scmf_server* sv) {
       const auto d = sv->doomed.back();
       sv->doomed.pop_back();
       lk.unlock();
-      scmf_destroy(d.first);
+      {
+        std::lock_guard<std::mutex> cl(scmf_batch_tickets::capture_mutex());
+        scmf_destroy(d.first);
+      }
       lk.lock();
       sv->used -= d.second;
       sv->cv.notify_all();
@@ -175,14 +178,19 @@ void worker_loop(scmf_server* sv) {
     lk.unlock();
     // test.cpp:19-23: create, initial configuration, gridding, omega
     scmf_sys* sys = nullptr;
-    int rc = scmf_create(&sys, &j.desc.d, &j.opts);
-    if (!rc) rc = scmf_grid_density(sys);
-    if (!rc) rc = scmf_update_omega(sys);
-    if (!rc) rc = scmf_sync(sys);
-    const std::string err = rc ? scmf_last_error() : std::string();
-    if (rc && sys) {
-      scmf_destroy(sys);
-      sys = nullptr;
+    int rc;
+    std::string err;
+    {
+      std::lock_guard<std::mutex> cl(scmf_batch_tickets::capture_mutex());
+      rc = scmf_create(&sys, &j.desc.d, &j.opts);
+      if (!rc) rc = scmf_grid_density(sys);
+      if (!rc) rc = scmf_update_omega(sys);
+      if (!rc) rc = scmf_sync(sys);
+      if (rc) err = scmf_last_error();
+      if (rc && sys) {
+        scmf_destroy(sys);
+        sys = nullptr;
+      }
     }
     lk.lock();
     sv->creating -= 1;
