@@ -588,28 +588,39 @@ size_t sched_pre_lds_bytes(int L, int nmax) {
 // indices by round.  Records are interleaved by groups of 8 slots (the
 // chains of one sweep wavefront at 8 lanes per chain): entry k of slot s is
 // byte ((s / 8) * stride + k) * 8 + s % 8, so a sweep wavefront reads its
-// chains' records with contiguous loads.  The lane arrays of this kernel
-// live in LDS interleaved by lane (entry k of lane t at k * 64 + t).
+// chains' records with contiguous loads.  The records of a workgroup's 64
+// chains (8 groups) are one contiguous run of 64 * stride bytes: they are
+// made in LDS in that layout and stored with 16-byte stores (the buffer
+// holds whole runs).  Beside them each lane has one array in LDS, interleaved
+// by lane (entry k of lane t at k * 64 + t): the round of each bead during
+// the scan, then the round of each move.  During the scan the round of each
+// move is kept in the record's list entries, which the list later fills, so
+// the kernel needs 3N + 5 bytes a chain and runs 12 waves per CU (more LDS
+// a chain measured slower: fewer waves hide less of the scan's latency).
 constexpr int SCHED_LANES = 64;
-size_t schedule_lds_bytes(int nmax) { return (size_t)(3 * nmax + 4) * SCHED_LANES; }
+size_t schedule_lds_bytes(int nmax) { return (size_t)(3 * nmax + 5) * SCHED_LANES; }
+size_t schedule_record_bytes(int nmax, int64_t chains) {
+  return (size_t)(2 * nmax + 3) * SCHED_LANES * (size_t)((chains + SCHED_LANES - 1) / SCHED_LANES);
+}
 
 __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restrict__ Ps, int nmax,
                                                           int cap) {
-  extern __shared__ uint8_t sh[];
+  extern __shared__ __align__(16) uint8_t sh[];
   KPC& kp = param(Ps, 0);
   const int t = threadIdx.x;
   const int64_t chain = (int64_t)blockIdx.x * SCHED_LANES + t;
-  if (chain >= kp.n_chains) return;
+  const bool active = chain < kp.n_chains;
   const uint32_t sweep = ((const uint32_t*)kp.ctr)[0];
-  const int p = arch_of_chain(kp, chain);
-  const int length = kp.len[p];
-  const uint32_t id = gptr((const uint32_t*)kp.slot_chain)[chain];
-  uint8_t* const lev = sh + t;                           // [nmax] round of each move
-  uint8_t* const W = lev + nmax * SCHED_LANES;           // [nmax + 2], W[b + 1] = round of bead b
-  uint8_t* const start = W + (nmax + 2) * SCHED_LANES;   // [nmax + 2]
-  auto at = [](uint8_t* a, int k) -> uint8_t& { return a[k * SCHED_LANES]; };
-  SCMF_GLOBAL uint8_t* __restrict__ rec = gptr(kp.sched);
+  const int p = active ? arch_of_chain(kp, chain) : 0;
+  const int length = active ? kp.len[p] : 0;
+  const uint32_t id = active ? gptr((const uint32_t*)kp.slot_chain)[chain] : 0u;
   const int stride = kp.sched_stride;
+  // this lane's record (entry k at byte 8 k), then its lane array
+  uint8_t* const rl = sh + (t >> 3) * stride * 8 + (t & 7);
+  uint8_t* const W = sh + SCHED_LANES * stride + t;  // [nmax + 2], W[b + 1] = round of bead b
+  auto at = [](uint8_t* a, int k) -> uint8_t& { return a[k * SCHED_LANES]; };
+  auto st = [&](int k) -> uint8_t& { return rl[8 * (1 + k)]; };         // start[k], k <= N + 1
+  auto lv = [&](int i) -> uint8_t& { return rl[8 * (length + 3 + i)]; };  // list entry i
   u32x4 blk{};
   auto pick = [&](int i) {  // one pick block per two moves
     if ((i & 1) == 0) blk = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i >> 1, SLOT_PICK);
@@ -617,7 +628,7 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restr
   };
   for (int k = 0; k <= length + 1; ++k) {
     at(W, k) = 0;
-    at(start, k) = 0;
+    st(k) = 0;
   }
   // levels, the next move's pick drawn while this move's LDS reads are in flight
   const bool linear = kp.linear[p] != 0;
@@ -636,25 +647,31 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restr
       l += 1;
       if (i + 1 < length) b = pick(i + 1);
     }
-    while (at(start, l + 1) >= cap) ++l;
+    while (st(l + 1) >= cap) ++l;
     at(W, cur + 1) = (uint8_t)l;
-    at(lev, i) = (uint8_t)l;
-    at(start, l + 1) = (uint8_t)(at(start, l + 1) + 1);
+    lv(i) = (uint8_t)l;  // (the round of move i, until the list is made)
+    st(l + 1) = (uint8_t)(st(l + 1) + 1);
     maxl = max(maxl, l);
   }
-  for (int l = 1; l <= maxl; ++l) {
-    at(start, l + 1) = (uint8_t)(at(start, l + 1) + at(start, l));
-    at(W, l) = 0;  // fill counters from here on
-  }
-  rec[sched_index(chain, stride, 0)] = (uint8_t)maxl;
-  for (int k = 0; k <= length + 1; ++k) rec[sched_index(chain, stride, 1 + k)] = at(start, k);
+  for (int l = 1; l <= maxl; ++l) st(l + 1) = (uint8_t)(st(l + 1) + st(l));
+  rl[0] = (uint8_t)maxl;
+  // the moves by round: the rounds of the moves to W, then start[l] serves
+  // as round l's next position (ending at the old start[l + 1]), restored
+  // after by shifting up
+  for (int i = 0; i < length; ++i) at(W, i) = lv(i);
   for (int i = 0; i < length; ++i) {
-    const int l = at(lev, i);
-    const int k = at(W, l);
-    at(W, l) = (uint8_t)(k + 1);
-    const int pos = at(start, l) + k;
-    rec[sched_index(chain, stride, length + 3 + pos)] = (uint8_t)i;
+    const int l = at(W, i);
+    const int pos = st(l);
+    st(l) = (uint8_t)(pos + 1);
+    lv(pos) = (uint8_t)i;
   }
+  for (int l = maxl; l >= 1; --l) st(l + 1) = st(l);
+  if (length > 0) st(1) = 0;
+  __syncthreads();
+  // the workgroup's run of records (inactive lanes' bytes are never read)
+  uint4* __restrict__ dst = (uint4*)(gptr(kp.sched) + (int64_t)blockIdx.x * SCHED_LANES * stride);
+  const uint4* src = (const uint4*)sh;
+  for (int q = t; q < 4 * stride; q += SCHED_LANES) dst[q] = src[q];
 }
 
 // Clear each system's counts (a kernel, not a memset, so it can sit in a graph).
@@ -1768,8 +1785,7 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
     return bail(rc);
   if (s->pre_sched) {
     s->kp.sched_stride = 2 * s->nmax + 3;
-    if ((rc = alloc((void**)&s->sched,
-                    (size_t)s->kp.sched_stride * 8 * (size_t)((s->n_chains + 7) / 8))))
+    if ((rc = alloc((void**)&s->sched, schedule_record_bytes(s->nmax, s->n_chains))))
       return bail(rc);
     s->kp.sched = s->sched;
   }
@@ -2763,7 +2779,7 @@ void fill_summary(const scmf_desc* d, scmf_summary* out) {
   uint64_t fixed = 12ull * B4 + (uint64_t)beads + 8ull * chains + 4ull * TC +
                    8ull * ((chains + BLOCK - 1) / BLOCK) + 4096;
   if (nmax <= 255 && chains >= 65536)  // schedule records
-    fixed += (uint64_t)(2 * nmax + 3) * 8ull * (uint64_t)((chains + 7) / 8);
+    fixed += (uint64_t)schedule_record_bytes(nmax, chains);
   if (beads >= ((int64_t)1 << 23)) {  // automatic ordering: second position buffer, cells,
     fixed += 16ull * B4 + 16ull * chains;  // keys and maps
     // window images and tile counters of the tiled gridding, from the same plan
