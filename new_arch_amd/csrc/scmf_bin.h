@@ -32,6 +32,19 @@
 // rounding in fp64 nor, for fp32 r, in fp32; so for 0 <= r < L (1 - 2^-50)
 // the reference's cell is trunc(r 2^k) whenever that is below n.  The test
 // returns false otherwise and the caller takes fold.
+//
+// Power-of-two cells in a power-of-two box (L = 2^j, n = 2^(j+k)): r/L, the
+// integer k, L k and the division by d are exact, so for r >= 0 the
+// reference's cell is floor(r/d) mod n = floor(r 2^k) & (n - 1) (with
+// r - L k exact: a multiple of r's last bit below 2^j).  For r < 0 only the
+// sum s = k L + r, 0 < s <= L, can round, by at most 2^(j-53), and rounding
+// never crosses a multiple of d from above (those are representable); so the
+// cell is again floor(r 2^k) & (n - 1) unless r/d lies within n 2^-53 below
+// an integer.  fp32 r: s is exact when |r| >= 2^(j-29) (s is a multiple of
+// r's last bit, 2^(e-23), and below 2^(j+1)).  fp64 r (an old + d sum): the
+// fractional part f of r/d must stay below 1 - n 2^-52.  Both need |r/d| <
+// 2^30, inside the reference's own integer range.  Beads outside the box
+// (unwrapped coordinates) then take this path instead of fold.
 #pragma once
 #include <stdint.h>
 
@@ -49,6 +62,9 @@ struct Axis {
   float invd_f, lhi_f;              // 1/d and lhi rounded to fp32 (used when pow2)
   int n;
   int pow2;                         // d is an exact power of two
+  int wrap2;                        // and L = n d with n a power of two
+  float rneg_f;                     // -2^(j-29): fp32 r <= this bins exactly (wrap2)
+  double gw1;                       // 1 - n 2^-52: bound on frac(r/d) for fp64 r < 0 (wrap2)
 };
 
 // The reference's computation, verbatim.  Coordinates it cannot bin
@@ -111,6 +127,12 @@ SCMF_HD int fold(double r, const Axis a) {
 // Power-of-two cells, fp32 coordinate: *i = the reference cell when true.
 // (Every fp32 r < (float)lhi is below lhi itself.)
 SCMF_HD bool fold_pow2(float r, const Axis a, int* i) {
+  if (a.wrap2) {
+    const float q = __builtin_floorf(r * a.invd_f);
+    const bool ok = (r >= 0.0f || r <= a.rneg_f) && __builtin_fabsf(q) < 1073741824.0f;
+    *i = (int)(ok ? q : 0.0f) & (a.n - 1);
+    return ok;
+  }
   const bool in = r >= 0.0f && r < a.lhi_f;
   const int c = (int)((in ? r : 0.0f) * a.invd_f);
   *i = c;
@@ -119,6 +141,13 @@ SCMF_HD bool fold_pow2(float r, const Axis a, int* i) {
 
 // Power-of-two cells, fp64 coordinate.
 SCMF_HD bool fold_pow2(double r, const Axis a, int* i) {
+  if (a.wrap2) {
+    const double q = r * a.invd;
+    const double c = __builtin_floor(q);
+    const bool ok = (r >= 0.0 || q - c < a.gw1) && __builtin_fabs(c) < 1073741824.0;
+    *i = (int)(ok ? c : 0.0) & (a.n - 1);
+    return ok;
+  }
   const bool in = r >= 0.0 && r < a.lhi;
   const int c = (int)((in ? r : 0.0) * a.invd);
   *i = c;
@@ -141,6 +170,9 @@ SCMF_HD Axis make_axis(double L, int n) {
   a.pow2 = (m == 0.5 && e > -120 && e < 120) ? 1 : 0;
   a.invd_f = (float)a.invd;
   a.lhi_f = (float)a.lhi;
+  a.wrap2 = a.pow2 && n > 0 && (n & (n - 1)) == 0 && a.d * n == L ? 1 : 0;
+  a.rneg_f = a.wrap2 ? -(float)(L * 0x1p-29) : 0.0f;
+  a.gw1 = 1.0 - 0x1p-52 * (double)n;
   return a;
 }
 

@@ -146,6 +146,9 @@ __device__ __forceinline__ Axis axis(const KP& kp, int a) {
   r.lhi_f = kp.ax[a].lhi_f;
   r.n = kp.ax[a].n;
   r.pow2 = kp.ax[a].pow2;
+  r.wrap2 = kp.ax[a].wrap2;
+  r.rneg_f = kp.ax[a].rneg_f;
+  r.gw1 = kp.ax[a].gw1;
   return r;
 }
 

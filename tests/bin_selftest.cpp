@@ -39,7 +39,7 @@ int main(int argc, char** argv) {
     }
   };
   for (long long i = 0; i < count; ++i) {
-    const int kind = (int)(i % 11);
+    const int kind = (int)(i % 14);
     double r;
     if (kind == 0) {
       r = (U(rng) * 2 - 1) * 50 * L;                    // unwrapped coordinates
@@ -83,6 +83,23 @@ int main(int argc, char** argv) {
       const int s = (int)(U(rng) * 7) - 3;
       for (int t = 0; t < abs(s); ++t) f = nextafterf(f, s > 0 ? INFINITY : -INFINITY);
       r = f;
+    } else if (kind == 11) {                            // negative fp32 cell edge + tiny fp32 step
+      const long long k = (long long)(U(rng) * 3 * n) + 1;
+      const float x = (float)(-k * a.d);
+      const float dx = (float)((U(rng) * 2 - 1) * a.d * ldexp(1.0, -(int)(20 + U(rng) * 45)));
+      r = (double)x + (double)dx;
+    } else if (kind == 12) {                            // fp32 around -L 2^-29 and tiny negatives
+      if (U(rng) < 0.5) {
+        float f = (float)(-L * ldexp(1.0, -29));
+        const int s = (int)(U(rng) * 9) - 4;
+        for (int t = 0; t < abs(s); ++t) f = nextafterf(f, s > 0 ? INFINITY : -INFINITY);
+        r = f;
+      } else {
+        r = (float)(-L * ldexp(U(rng) + 0.5, -(int)(25 + U(rng) * 120)));
+      }
+    } else if (kind == 13) {                            // fp64 just below a negative cell edge
+      const long long k = (long long)(U(rng) * 3 * n);
+      r = -k * a.d - a.d * ldexp(U(rng), -(int)(38 + U(rng) * 20));
     } else {                                            // tiny and signed zeros
       const double v[6] = {0.0, -0.0, 1e-300, -1e-300, 5e-324, -5e-324};
       r = v[i % 6] * (U(rng) < 0.5 ? 1 : 1e10);
