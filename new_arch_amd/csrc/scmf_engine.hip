@@ -603,6 +603,14 @@ size_t schedule_record_bytes(int nmax, int64_t chains) {
   return (size_t)(2 * nmax + 3) * SCHED_LANES * (size_t)((chains + SCHED_LANES - 1) / SCHED_LANES);
 }
 
+// CAPLOG > 0 (chains of at most 64 beads, rounds of at most cap = 2^CAPLOG
+// moves): the number of moves in each round is kept in registers as
+// CAPLOG + 1 bit-planes of 64 bits (bit l - 1 of plane j = bit j of round
+// l's count), so a move finds the first round with room by a bit scan of the
+// top plane (the full rounds) instead of reading round counts from LDS; the
+// counts go to the record after the scan.  (Rounds used are always 1 .. max,
+// at most one per move, so 64 bits hold them.)  CAPLOG = 0: counts in LDS.
+template <int CAPLOG>
 __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restrict__ Ps, int nmax,
                                                           int cap) {
   extern __shared__ __align__(16) uint8_t sh[];
@@ -634,6 +642,7 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restr
   const bool linear = kp.linear[p] != 0;
   int maxl = 0;
   int b = length > 0 ? pick(0) : 0;
+  uint64_t plane[CAPLOG + 1] = {};
   for (int i = 0; i < length; ++i) {
     const int cur = b;
     int l;
@@ -647,11 +656,30 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restr
       l += 1;
       if (i + 1 < length) b = pick(i + 1);
     }
-    while (st(l + 1) >= cap) ++l;
+    if constexpr (CAPLOG > 0) {
+      l = __builtin_ctzll(~plane[CAPLOG] & (~0ull << (l - 1))) + 1;
+      uint64_t carry = 1ull << (l - 1);
+#pragma unroll
+      for (int j = 0; j <= CAPLOG; ++j) {
+        const uint64_t t = plane[j] & carry;
+        plane[j] ^= carry;
+        carry = t;
+      }
+    } else {
+      while (st(l + 1) >= cap) ++l;
+      st(l + 1) = (uint8_t)(st(l + 1) + 1);
+    }
     at(W, cur + 1) = (uint8_t)l;
     lv(i) = (uint8_t)l;  // (the round of move i, until the list is made)
-    st(l + 1) = (uint8_t)(st(l + 1) + 1);
     maxl = max(maxl, l);
+  }
+  if constexpr (CAPLOG > 0) {
+    for (int l = 1; l <= maxl; ++l) {
+      int n = 0;
+#pragma unroll
+      for (int j = 0; j <= CAPLOG; ++j) n |= (int)((plane[j] >> (l - 1)) & 1u) << j;
+      st(l + 1) = (uint8_t)n;
+    }
   }
   for (int l = 1; l <= maxl; ++l) st(l + 1) = (uint8_t)(st(l + 1) + st(l));
   rl[0] = (uint8_t)maxl;
@@ -672,6 +700,15 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restr
   uint4* __restrict__ dst = (uint4*)(gptr(kp.sched) + (int64_t)blockIdx.x * SCHED_LANES * stride);
   const uint4* src = (const uint4*)sh;
   for (int q = t; q < 4 * stride; q += SCHED_LANES) dst[q] = src[q];
+}
+
+// The schedule kernel for chains of at most nmax beads and rounds of cap moves.
+using ScheduleFn = void (*)(const KParams*, int, int);
+ScheduleFn schedule_kernel(int nmax, int cap) {
+  if (nmax <= 64 && cap == 2) return k_schedule<1>;
+  if (nmax <= 64 && cap == 4) return k_schedule<2>;
+  if (nmax <= 64 && cap == 8) return k_schedule<3>;
+  return k_schedule<0>;
 }
 
 // Clear each system's counts (a kernel, not a memset, so it can sit in a graph).
@@ -1442,7 +1479,8 @@ int launch_sweep(scmf_sys* s) {
     const int G = 64 / v->L;
     const dim3 grid((unsigned)((s->n_chains + G - 1) / G));
     if (s->pre_sched)
-      hipLaunchKernelGGL(k_schedule, dim3((unsigned)((s->n_chains + SCHED_LANES - 1) / SCHED_LANES)),
+      hipLaunchKernelGGL(schedule_kernel(s->nmax, v->L),
+                         dim3((unsigned)((s->n_chains + SCHED_LANES - 1) / SCHED_LANES)),
                          dim3(SCHED_LANES), schedule_lds_bytes(s->nmax), s->stream,
                          (const KParams*)s->d_kp, s->nmax, v->L);
     const KParams* kpp = s->d_kp;
