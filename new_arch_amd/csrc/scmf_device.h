@@ -76,7 +76,7 @@ struct KParams {
   uint32_t* ctr;                 // [0] stream sweep word of the next sweep
   uint32_t* cells;               // packed cell of every stored bead, written by the sweep
                                  // (x | y << 10 | z << 20), or null
-  uint8_t* sched;                // per slot: the sweep's move schedule made by k_schedule
+  uint8_t* sched;                // per slot: the sweep's move schedule made by the schedule kernels
   int sched_stride;              // bytes per slot record
   uint32_t stop;                 // batches: the member does no work once its sweep word
                                  // reaches this (NO_STOP: never); ctr[1] says whether the

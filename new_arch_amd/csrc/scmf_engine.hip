@@ -397,11 +397,11 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
 // around, then per-round fill counts); start[N+2] (moves per round during
 // the scan, then the first list entry of each round); list[N] (moves by
 // round); rnd[N] (the round of each move).  With a schedule made by
-// k_schedule only start and list are kept.
+// the schedule kernels only start and list are kept.
 // (At most 96 VGPRs, i.e. five waves per SIMD: the LDS tiles allow about
 // as many, and the move is issue-bound, so occupancy is worth the spills.)
 // Byte k of the move-schedule record of a slot (records interleaved by
-// groups of 8 slots; made by k_schedule below).
+// groups of 8 slots; made by k_schedule_reg or k_schedule_lds below).
 __device__ __forceinline__ int64_t sched_index(int64_t slot, int stride, int k) {
   return ((slot >> 3) * (int64_t)stride + k) * 8 + (slot & 7);
 }
@@ -425,7 +425,7 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
   float* sy = lds + nmax * G;
   float* sz = lds + 2 * nmax * G;
   I* book = (I*)(lds + 3 * nmax * G);
-  // A schedule made by k_schedule (single systems) needs start and list
+  // A schedule made by the schedule kernels (single systems) needs start and list
   // only; otherwise the chain's lanes make it here from lev and W.
   const SCMF_GLOBAL uint8_t* __restrict__ sched = gptr((const uint8_t*)kp.sched);
   const bool pre = sched != nullptr;
@@ -572,7 +572,7 @@ size_t sched_lds_bytes(int nmax) {
   return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (5 * nmax + 4) * sizeof(I);
 }
 
-// The same kernel reading a schedule made by k_schedule.
+// The same kernel reading a schedule made by the schedule kernels.
 size_t sched_pre_lds_bytes(int L, int nmax) {
   const int G = 64 / L;
   return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (2 * nmax + 4);
@@ -584,35 +584,120 @@ size_t sched_pre_lds_bytes(int L, int nmax) {
 // chain).  The picks depend on the streams only, so the schedule does not
 // wait for positions.  Record of a slot (uint8, chains of at most 255
 // beads): entry 0 the number of rounds, entries 1 .. N+2 start[0 .. N+1]
-// (the first list entry of each round), entries N+3 .. 2N+2 the move
-// indices by round.  Records are interleaved by groups of 8 slots (the
-// chains of one sweep wavefront at 8 lanes per chain): entry k of slot s is
-// byte ((s / 8) * stride + k) * 8 + s % 8, so a sweep wavefront reads its
-// chains' records with contiguous loads.  The records of a workgroup's 64
-// chains (8 groups) are one contiguous run of 64 * stride bytes: they are
-// made in LDS in that layout and stored with 16-byte stores (the buffer
-// holds whole runs).  Beside them each lane has one array in LDS, interleaved
-// by lane (entry k of lane t at k * 64 + t): the round of each bead during
-// the scan, then the round of each move.  During the scan the round of each
-// move is kept in the record's list entries, which the list later fills, so
-// the kernel needs 3N + 5 bytes a chain and runs 12 waves per CU (more LDS
-// a chain measured slower: fewer waves hide less of the scan's latency).
+// (the first list entry of each round; zero after the last round), entries
+// N+3 .. 2N+2 the move indices by round.  Records are interleaved by groups
+// of 8 slots (the chains of one sweep wavefront at 8 lanes per chain): entry
+// k of slot s is byte ((s / 8) * stride + k) * 8 + s % 8, so a sweep
+// wavefront reads its chains' records with contiguous loads.  The lane
+// arrays of these kernels live in LDS interleaved by lane (entry k of lane t
+// at k * 64 + t).  The scan is bound by the latency of its LDS round trips,
+// so LDS per chain (waves per CU) decides its speed.
 constexpr int SCHED_LANES = 64;
-size_t schedule_lds_bytes(int nmax) { return (size_t)(3 * nmax + 5) * SCHED_LANES; }
 size_t schedule_record_bytes(int nmax, int64_t chains) {
   return (size_t)(2 * nmax + 3) * SCHED_LANES * (size_t)((chains + SCHED_LANES - 1) / SCHED_LANES);
 }
+// Whether the register-count kernel takes chains of nmax beads in rounds of cap moves.
+bool schedule_in_registers(int nmax, int cap) {
+  return nmax <= 64 && (cap == 2 || cap == 4 || cap == 8);
+}
+size_t schedule_lds_bytes(int nmax, int cap) {
+  return (size_t)(schedule_in_registers(nmax, cap) ? 2 * nmax + 2 : 3 * nmax + 5) * SCHED_LANES;
+}
 
-// CAPLOG > 0 (chains of at most 64 beads, rounds of at most cap = 2^CAPLOG
-// moves): the number of moves in each round is kept in registers as
-// CAPLOG + 1 bit-planes of 64 bits (bit l - 1 of plane j = bit j of round
-// l's count), so a move finds the first round with room by a bit scan of the
-// top plane (the full rounds) instead of reading round counts from LDS; the
-// counts go to the record after the scan.  (Rounds used are always 1 .. max,
-// at most one per move, so 64 bits hold them.)  CAPLOG = 0: counts in LDS.
+// Chains of at most 64 beads, rounds of at most cap = 2^CAPLOG moves.  The
+// number of moves in each round is kept in registers as CAPLOG + 1
+// bit-planes of 64 bits (bit l - 1 of plane j = bit j of round l's count),
+// so a move finds the first round with room by a bit scan of the top plane
+// (the full rounds) and counts itself by a ripple over the planes: the scan
+// waits for LDS once per move (its neighbours' rounds).  The rounds used
+// are always 1 .. max, at most one per move, so 64 bits hold them.  LDS
+// holds the round of each bead and of each move (2N + 2 bytes a chain); the
+// record goes to memory byte by byte (staging it in LDS for 16-byte stores
+// costs waves, and measured slower).
 template <int CAPLOG>
-__global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restrict__ Ps, int nmax,
-                                                          int cap) {
+__global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __restrict__ Ps,
+                                                              int nmax) {
+  extern __shared__ uint8_t sh[];
+  KPC& kp = param(Ps, 0);
+  const int t = threadIdx.x;
+  const int64_t chain = (int64_t)blockIdx.x * SCHED_LANES + t;
+  if (chain >= kp.n_chains) return;
+  const uint32_t sweep = ((const uint32_t*)kp.ctr)[0];
+  const int p = arch_of_chain(kp, chain);
+  const int length = kp.len[p];
+  const uint32_t id = gptr((const uint32_t*)kp.slot_chain)[chain];
+  const int stride = kp.sched_stride;
+  uint8_t* const W = sh + t;                            // [nmax + 2], W[b + 1] = round of bead b
+  uint8_t* const lev = W + (nmax + 2) * SCHED_LANES;    // [nmax] round of each move
+  auto at = [](uint8_t* a, int k) -> uint8_t& { return a[k * SCHED_LANES]; };
+  SCMF_GLOBAL uint8_t* __restrict__ rec = gptr(kp.sched);
+  u32x4 blk{};
+  auto pick = [&](int i) {  // one pick block per two moves
+    if ((i & 1) == 0) blk = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i >> 1, SLOT_PICK);
+    return pick_bead(pick_u01(blk, (uint32_t)i), length);
+  };
+  for (int k = 0; k <= length + 1; ++k) at(W, k) = 0;
+  // rounds, the next move's pick drawn while this move's LDS reads are in flight
+  const bool linear = kp.linear[p] != 0;
+  int maxl = 0;
+  int b = length > 0 ? pick(0) : 0;
+  uint64_t plane[CAPLOG + 1] = {};
+  for (int i = 0; i < length; ++i) {
+    const int cur = b;
+    int l;
+    if (linear) {
+      const int w0 = at(W, cur), w1 = at(W, cur + 1), w2 = at(W, cur + 2);
+      if (i + 1 < length) b = pick(i + 1);
+      l = max(max(w0, w1), w2) + 1;
+    } else {
+      l = at(W, cur + 1);
+      for_neighbours(kp, p, cur, length, [&](int j) { l = max(l, (int)at(W, j + 1)); });
+      l += 1;
+      if (i + 1 < length) b = pick(i + 1);
+    }
+    l = __builtin_ctzll(~plane[CAPLOG] & (~0ull << (l - 1))) + 1;
+    uint64_t carry = 1ull << (l - 1);
+#pragma unroll
+    for (int j = 0; j <= CAPLOG; ++j) {
+      const uint64_t c = plane[j] & carry;
+      plane[j] ^= carry;
+      carry = c;
+    }
+    at(W, cur + 1) = (uint8_t)l;
+    at(lev, i) = (uint8_t)l;
+    maxl = max(maxl, l);
+  }
+  // round starts (start[0] = start[1] = 0, prefix sums through the last
+  // round, zero after it); W[l] becomes round l's next list position
+  rec[sched_index(chain, stride, 0)] = (uint8_t)maxl;
+  rec[sched_index(chain, stride, 1)] = 0;
+  rec[sched_index(chain, stride, 2)] = 0;
+  int run = 0;
+  for (int l = 1; l <= length; ++l) {
+    int n = 0;
+#pragma unroll
+    for (int j = 0; j <= CAPLOG; ++j) n |= (int)((plane[j] >> (l - 1)) & 1u) << j;
+    if (l <= maxl) at(W, l) = (uint8_t)run;
+    run += n;
+    rec[sched_index(chain, stride, 2 + l)] = (uint8_t)(l <= maxl ? run : 0);
+  }
+  for (int i = 0; i < length; ++i) {
+    const int l = at(lev, i);
+    const int pos = at(W, l);
+    at(W, l) = (uint8_t)(pos + 1);
+    rec[sched_index(chain, stride, length + 3 + pos)] = (uint8_t)i;
+  }
+}
+
+// Any chain length up to 255 and any cap: the round counts live in LDS, in
+// the record's start entries.  The records of a workgroup's 64 chains (8
+// groups) are one contiguous run of 64 * stride bytes: they are made in LDS
+// in that layout and stored with 16-byte stores (the buffer holds whole
+// runs).  During the scan the round of each move waits in the record's list
+// entries, which the list later fills, and the round of each bead in a lane
+// array, which then takes the moves' rounds: 3N + 5 bytes a chain.
+__global__ void __launch_bounds__(SCHED_LANES) k_schedule_lds(const KParams* __restrict__ Ps,
+                                                              int nmax, int cap) {
   extern __shared__ __align__(16) uint8_t sh[];
   KPC& kp = param(Ps, 0);
   const int t = threadIdx.x;
@@ -638,11 +723,10 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restr
     at(W, k) = 0;
     st(k) = 0;
   }
-  // levels, the next move's pick drawn while this move's LDS reads are in flight
+  // rounds, the next move's pick drawn while this move's LDS reads are in flight
   const bool linear = kp.linear[p] != 0;
   int maxl = 0;
   int b = length > 0 ? pick(0) : 0;
-  uint64_t plane[CAPLOG + 1] = {};
   for (int i = 0; i < length; ++i) {
     const int cur = b;
     int l;
@@ -656,30 +740,11 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restr
       l += 1;
       if (i + 1 < length) b = pick(i + 1);
     }
-    if constexpr (CAPLOG > 0) {
-      l = __builtin_ctzll(~plane[CAPLOG] & (~0ull << (l - 1))) + 1;
-      uint64_t carry = 1ull << (l - 1);
-#pragma unroll
-      for (int j = 0; j <= CAPLOG; ++j) {
-        const uint64_t t = plane[j] & carry;
-        plane[j] ^= carry;
-        carry = t;
-      }
-    } else {
-      while (st(l + 1) >= cap) ++l;
-      st(l + 1) = (uint8_t)(st(l + 1) + 1);
-    }
+    while (st(l + 1) >= cap) ++l;
+    st(l + 1) = (uint8_t)(st(l + 1) + 1);
     at(W, cur + 1) = (uint8_t)l;
     lv(i) = (uint8_t)l;  // (the round of move i, until the list is made)
     maxl = max(maxl, l);
-  }
-  if constexpr (CAPLOG > 0) {
-    for (int l = 1; l <= maxl; ++l) {
-      int n = 0;
-#pragma unroll
-      for (int j = 0; j <= CAPLOG; ++j) n |= (int)((plane[j] >> (l - 1)) & 1u) << j;
-      st(l + 1) = (uint8_t)n;
-    }
   }
   for (int l = 1; l <= maxl; ++l) st(l + 1) = (uint8_t)(st(l + 1) + st(l));
   rl[0] = (uint8_t)maxl;
@@ -702,14 +767,6 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule(const KParams* __restr
   for (int q = t; q < 4 * stride; q += SCHED_LANES) dst[q] = src[q];
 }
 
-// The schedule kernel for chains of at most nmax beads and rounds of cap moves.
-using ScheduleFn = void (*)(const KParams*, int, int);
-ScheduleFn schedule_kernel(int nmax, int cap) {
-  if (nmax <= 64 && cap == 2) return k_schedule<1>;
-  if (nmax <= 64 && cap == 4) return k_schedule<2>;
-  if (nmax <= 64 && cap == 8) return k_schedule<3>;
-  return k_schedule<0>;
-}
 
 // Clear each system's counts (a kernel, not a memset, so it can sit in a graph).
 __global__ void __launch_bounds__(BLOCK) k_zero_counts(const KParams* __restrict__ Ps,
@@ -1194,7 +1251,7 @@ struct scmf_sys {
   float* z2 = nullptr;
   uint16_t* images = nullptr;  // window images of the tiled gridding
   uint32_t* cells = nullptr;    // packed bead cells written by the sweep (tiled systems)
-  uint8_t* sched = nullptr;     // move schedules made by k_schedule (chains of <= 255 beads)
+  uint8_t* sched = nullptr;     // move schedules made by the schedule kernels (chains of <= 255 beads)
   bool pre_sched = false;       // the scheduled sweep shapes read them
   bool cells_fresh = false;     // cells match the positions (the last change was such a sweep)
   unsigned long long* d_fallback = nullptr;
@@ -1469,6 +1526,24 @@ int choose_sweep(scmf_sys* s) {
   return SCMF_OK;
 }
 
+// k_schedule_reg when its register counts hold the rounds, else k_schedule_lds.
+int launch_schedule(scmf_sys* s, int cap) {
+  const dim3 grid((unsigned)((s->n_chains + SCHED_LANES - 1) / SCHED_LANES));
+  const size_t lds = schedule_lds_bytes(s->nmax, cap);
+  const KParams* kp = s->d_kp;
+  if (schedule_in_registers(s->nmax, cap)) {
+    const void* fn = cap == 2 ? (const void*)k_schedule_reg<1>
+                   : cap == 4 ? (const void*)k_schedule_reg<2> : (const void*)k_schedule_reg<3>;
+    void* args[] = {(void*)&kp, (void*)&s->nmax};
+    HIP_TRY(hipLaunchKernel(fn, grid, dim3(SCHED_LANES), args, lds, s->stream));
+  } else {
+    void* args[] = {(void*)&kp, (void*)&s->nmax, (void*)&cap};
+    HIP_TRY(hipLaunchKernel((const void*)k_schedule_lds, grid, dim3(SCHED_LANES), args, lds,
+                            s->stream));
+  }
+  return SCMF_OK;
+}
+
 int launch_sweep(scmf_sys* s) {
   const SweepVariant* v = s->variant;
   const BlockRef* none = nullptr;
@@ -1478,11 +1553,10 @@ int launch_sweep(scmf_sys* s) {
   } else {
     const int G = 64 / v->L;
     const dim3 grid((unsigned)((s->n_chains + G - 1) / G));
-    if (s->pre_sched)
-      hipLaunchKernelGGL(schedule_kernel(s->nmax, v->L),
-                         dim3((unsigned)((s->n_chains + SCHED_LANES - 1) / SCHED_LANES)),
-                         dim3(SCHED_LANES), schedule_lds_bytes(s->nmax), s->stream,
-                         (const KParams*)s->d_kp, s->nmax, v->L);
+    if (s->pre_sched) {
+      const int rc = launch_schedule(s, v->L);
+      if (rc != SCMF_OK) return rc;
+    }
     const KParams* kpp = s->d_kp;
     void* args[] = {(void*)&kpp, (void*)&none, (void*)&s->nmax};
     HIP_TRY(hipLaunchKernel(v->fn, grid, dim3(64), args, s->lds_bytes, s->stream));

@@ -270,10 +270,22 @@ def test_gridding_follows_every_position_change(tmp_path, L, lanes):
         g.close()
 
 
+@pytest.mark.parametrize("lanes", [2, 8])
+def test_schedule_kernel_long_chains(lanes):
+    """70,000 diblocks of 96 beads: chains longer than 64 take the schedule
+    kernel that counts each round's moves in LDS (k_schedule_lds, records
+    staged in LDS), with rounds of 2 or 8 moves.  Sweeps bit-exact against
+    the oracle."""
+    banner(f"schedule kernel, N=96, lanes {lanes}")
+    spec = W.tiny(n_chains=70_000, N=96, grid=(32, 32, 32), L=(16.0, 16.0, 16.0))
+    run_pair(spec, sweeps=2, seed=31, density_bits=32, lanes_per_chain=lanes)
+
+
 @pytest.mark.parametrize("lanes", [0, 4])
 def test_schedule_kernel_with_branched_chains(lanes):
     """66,000 chains (homopolymers, diblocks and 4-arm stars): enough chains
-    that the move schedules come from k_schedule, whose general neighbour
-    scan the stars take.  Sweeps bit-exact against the oracle."""
+    that the move schedules come from the schedule kernels (register round
+    counts: chains of at most 64 beads), whose general neighbour scan the
+    stars take.  Sweeps bit-exact against the oracle."""
     banner(f"schedule kernel, stars, lanes {lanes}")
     run_pair(W.mixed_arch(scale=550), sweeps=2, seed=16, density_bits=32, lanes_per_chain=lanes)
