@@ -423,8 +423,10 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
     from_cells = st.cell_grids - st0.cell_grids > 0
     ordering = None
     if period:
-        # one ordering and the gridding after it (which bins positions: the
-        # gridding of the scope table, 12 B a bead) timed on their own
+        # one ordering and the gridding after it (the cells move with the
+        # positions, so it reads cells), timed on their own; then the
+        # gridding of the scope table, binning positions (12 B a bead), after
+        # the positions went out and back in (which leaves no cells to read)
         g.enable_timing(False)
         g.sync()
         t = time.perf_counter()
@@ -433,11 +435,24 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
         t_sort = time.perf_counter() - t
         g.enable_timing(True)
         g.reset_timing()
+        cg0 = g.stats().cell_grids
+        g.grid_density()
+        g.sync()
+        t_after = g.timing().grid_ms
+        after_from_cells = g.stats().cell_grids > cg0
+        g.enable_timing(False)
+        pos = g.positions()
+        g.set_positions(pos)
+        del pos
+        g.enable_timing(True)
+        g.reset_timing()
         g.grid_density()
         g.sync()
         t_bin = g.timing().grid_ms
         ordering = {"period_steps": period, "in_timed_steps": int(st.sorts - st0.sorts),
                     "ordering_ms": round(t_sort * 1e3, 3),
+                    "gridding_after_ordering_ms": round(t_after, 4),
+                    "gridding_after_ordering_reads_cells": bool(after_from_cells),
                     "gridding_from_positions_ms": round(t_bin, 4)}
     host = None
     if with_host_copy:
@@ -566,7 +581,7 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
             roof(12 * B + s * TC, ordering["gridding_from_positions_ms"], None),
             kernel=f"k_zero_counts+k_grid_tiles(positions)+"
                    f"{'k_tiles_sum_pair' if spec.n_types == 2 else 'k_tiles_sum'}",
-            when="after each spatial ordering")
+            when="after any change of positions other than a sweep or an ordering")
     dtNxi, nref = spec.dtNxi, spec.nref
     A = dtNxi / nref
     roofline = dict(roof(sweep_bytes, tm.sweep_ms, traffic_of(sweep_keys)),

@@ -1251,6 +1251,7 @@ struct scmf_sys {
   float* z2 = nullptr;
   uint16_t* images = nullptr;  // window images of the tiled gridding
   uint32_t* cells = nullptr;    // packed bead cells written by the sweep (tiled systems)
+  uint32_t* cells2 = nullptr;   // the cells' second buffer, for the ordering's permutation
   uint8_t* sched = nullptr;     // move schedules made by the schedule kernels (chains of <= 255 beads)
   bool pre_sched = false;       // the scheduled sweep shapes read them
   bool cells_fresh = false;     // cells match the positions (the last change was such a sweep)
@@ -1720,7 +1721,9 @@ int alloc_sort_buffers(scmf_sys* s) {
     return rc;
   // the sweep stores bead cells for the next gridding where 10 bits per axis hold them
   if (s->kp.nx <= CELL_AXIS_MAX && s->kp.ny <= CELL_AXIS_MAX && s->kp.nz <= CELL_AXIS_MAX) {
-    if ((rc = A((void**)&s->cells, sizeof(uint32_t) * padded(B)))) return rc;
+    if ((rc = A((void**)&s->cells, sizeof(uint32_t) * padded(B))) ||
+        (rc = A((void**)&s->cells2, sizeof(uint32_t) * padded(B))))
+      return rc;
     HIP_TRY(hipMemsetAsync(s->cells, 0, sizeof(uint32_t) * padded(B), s->stream));
     s->kp.cells = s->cells;
     s->cells_fresh = false;
@@ -1757,19 +1760,27 @@ int sort_chains(scmf_sys* s) {
                      s->tfill, nk);
   hipLaunchKernelGGL(k_tile_scatter, dim3(grid_for(C)), dim3(256), 0, s->stream, C, s->keys,
                      s->tstart, s->tfill, s->new_slot);
+  // cells that match the positions move with them (a bead's cell does not
+  // depend on its slot), so the next gridding still reads cells
+  const bool carry = s->cells_fresh && s->cells2 != nullptr;
   hipLaunchKernelGGL(k_permute, dim3(grid_strided(s->n_beads)), dim3(256), 0, s->stream, s->kp,
-                     s->x, s->y, s->z, s->x2, s->y2, s->z2, s->new_slot, s->slot_chain,
-                     s->slot_chain2, s->chain_slot);
+                     s->x, s->y, s->z, s->x2, s->y2, s->z2, carry ? s->cells : nullptr,
+                     carry ? s->cells2 : nullptr, s->new_slot, s->slot_chain, s->slot_chain2,
+                     s->chain_slot);
   HIP_TRY(hipGetLastError());
   std::swap(s->x, s->x2);
   std::swap(s->y, s->y2);
   std::swap(s->z, s->z2);
   std::swap(s->slot_chain, s->slot_chain2);
+  if (carry) {
+    std::swap(s->cells, s->cells2);
+    s->kp.cells = s->cells;
+  }
   s->kp.slot_chain = s->slot_chain;
   s->kp.x = s->x;
   s->kp.y = s->y;
   s->kp.z = s->z;
-  s->cells_fresh = false;  // positions permuted, cells not
+  s->cells_fresh = carry;
   if ((rc = sync_kp(s))) return rc;
   s->sorted = true;
   s->n_sorts += 1;
@@ -1811,7 +1822,7 @@ int free_all(scmf_sys* s) {
                   (void*)s->tstart, (void*)s->tfill, (void*)s->new_slot, (void*)s->x2,
                   (void*)s->y2, (void*)s->z2, (void*)s->images, (void*)s->d_fallback,
                   (void*)s->d_kp, (void*)s->d_ctr, (void*)s->fft_buf, (void*)s->d_sq_sum,
-                  (void*)s->d_sq_cnt, (void*)s->cells, (void*)s->sched})
+                  (void*)s->d_sq_cnt, (void*)s->cells, (void*)s->cells2, (void*)s->sched})
     if (p) (void)hipFree(p);
   if (s->fft_ready) (void)hipfftDestroy(s->fft);
   for (Timed& t : s->pending) {
@@ -2892,8 +2903,8 @@ void fill_summary(const scmf_desc* d, scmf_summary* out) {
                    8ull * ((chains + BLOCK - 1) / BLOCK) + 4096;
   if (nmax <= 255 && chains >= 65536)  // schedule records
     fixed += (uint64_t)schedule_record_bytes(nmax, chains);
-  if (beads >= ((int64_t)1 << 23)) {  // automatic ordering: second position buffer, cells,
-    fixed += 16ull * B4 + 16ull * chains;  // keys and maps
+  if (beads >= ((int64_t)1 << 23)) {  // automatic ordering: second position buffer, two cell
+    fixed += 20ull * B4 + 16ull * chains;  // buffers, keys and maps
     // window images and tile counters of the tiled gridding, from the same plan
     std::vector<std::pair<int, int64_t>> lens;
     for (int p = 0; p < d->n_poly_types; ++p)
@@ -2905,7 +2916,7 @@ void fill_summary(const scmf_desc* d, scmf_summary* out) {
       fixed += 2ull * ((uint64_t)g.ntiles * (uint64_t)g.slot_elems + 64) +
                12ull * (uint64_t)d->n_poly_types * (uint64_t)(g.ntiles + 1);
   }
-  fixed += 26ull << 21;  // up to 26 device allocations, each rounded to 2 MiB
+  fixed += 27ull << 21;  // up to 27 device allocations, each rounded to 2 MiB
   out->device_bytes_16 = fixed + 4ull * ((TC + 1) / 2);
   out->device_bytes_32 = fixed + 4ull * TC;
 }

@@ -147,14 +147,16 @@ __global__ void __launch_bounds__(256) k_tile_scatter(int64_t n, const uint32_t*
   new_slot[s] = start[k] + wave_add_one(fill, k);
 }
 
-// Move every chain to its new slot (positions double-buffered) and update
-// the slot <-> chain maps.  One lane per bead; the slot of a bead follows
+// Move every chain to its new slot (positions double-buffered, and the bead
+// cells when they match the positions) and update the slot <-> chain maps.  One lane per bead; the slot of a bead follows
 // from its architecture block by one fp64 product, corrected by one step.
 __global__ void __launch_bounds__(256) k_permute(KParams kp, const float* __restrict__ x,
                                                  const float* __restrict__ y,
                                                  const float* __restrict__ z,
                                                  float* __restrict__ x2, float* __restrict__ y2,
                                                  float* __restrict__ z2,
+                                                 const uint32_t* __restrict__ cells,
+                                                 uint32_t* __restrict__ cells2,
                                                  const uint32_t* __restrict__ new_slot,
                                                  const uint32_t* __restrict__ slot_chain,
                                                  uint32_t* __restrict__ slot_chain2,
@@ -180,6 +182,7 @@ __global__ void __launch_bounds__(256) k_permute(KParams kp, const float* __rest
     x2[dst] = x[gb];
     y2[dst] = y[gb];
     z2[dst] = z[gb];
+    if (cells) cells2[dst] = cells[gb];
     if (m == 0) {
       const uint32_t c = slot_chain[s];
       slot_chain2[ns] = c;

@@ -224,11 +224,13 @@ def test_tiled_gridding_architecture_longer_than_type_table(bits):
 
 @pytest.mark.parametrize("L,lanes", [(4.0, 0), (3.3, 8)])
 def test_gridding_follows_every_position_change(tmp_path, L, lanes):
-    """A tiled system grids from the bead cells its last sweep stored; every
-    other change of positions (set_positions, ordering, checkpoint restore,
-    autotune) makes it bin positions again.  Counts equal the oracle's after
-    each, and cell_grids counts the griddings that read cells.  L = 4 gives
-    power-of-two cells, L = 3.3 the general binning in the sweep."""
+    """A tiled system grids from the bead cells its last sweep stored, also
+    after a spatial ordering (the cells move with the positions); every other
+    change of positions (set_positions, checkpoint restore, autotune) makes it
+    bin positions again, and so does an ordering of positions whose cells
+    are stale.  Counts equal the oracle's after each, and cell_grids counts
+    the griddings that read cells.  L = 4 gives power-of-two cells, L = 3.3
+    the general binning in the sweep."""
     banner(f"cells follow positions, L={L}")
     spec = W.tiny(n_chains=3000, N=32, grid=(32, 32, 32), L=(L, L, L))
     g = scmf.System(spec, make_opts(seed=17, density_bits=32, sort_every=1,
@@ -251,21 +253,22 @@ def test_gridding_follows_every_position_change(tmp_path, L, lanes):
         pos = g.positions()
         pos[::7] += np.float32(0.37)               # some beads leave their cells
         g.set_positions(pos)
+        g.sort_chains()                            # stale cells: not carried
         check(2)
         g.mc_sweep(1)
-        g.sort_chains()                            # slots permuted, cells not
-        check(2)
+        g.sort_chains()                            # slots permuted, cells with them
+        check(3)
         g.mc_sweep(1)
         ck = tmp_path / "c.ckpt"
         g.checkpoint(ck, 3)
         g.mc_sweep(2)
         assert g.restore(ck) == 3
-        check(2)
+        check(3)
         g.mc_sweep(1)
         tuned, _ = g.autotune(1)                   # tries every shape, restores positions
-        check(2)
+        check(3)
         g.mc_sweep(1)
-        check(3 if tuned >= 2 else 2)              # only the scheduled shapes store cells
+        check(4 if tuned >= 2 else 3)              # only the scheduled shapes store cells
     finally:
         g.close()
 
