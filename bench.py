@@ -332,7 +332,8 @@ def served(n_jobs, rate, seed=0, steps_per_tick=25, plain_sample=24):
             "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()), "aggregate_bead_moves_per_s": agg,
             "moves": int(moves), "span_s": t_last - t_first,
-            "gpu_busy_fraction": busy, "ticks": int(st.ticks), "rebuilds": int(st.rebuilds),
+            "gpu_busy_fraction": busy, "ticks": int(st.ticks),
+            "membership_changes": int(st.rebuilds), "graph_captures": int(st.captures),
             "steps_per_tick": steps_per_tick,
             "plain_one_at_a_time": plain,
             "served_over_plain": agg / plain["bead_moves_per_s"] if plain else None,

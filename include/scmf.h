@@ -281,6 +281,19 @@ int scmf_batch_sync(scmf_batch* batch);
 int scmf_batch_set_lanes(scmf_batch* batch, int32_t lanes_per_chain);
 void scmf_batch_destroy(scmf_batch* batch);
 
+/* The step graph is captured with room (members, workgroups, chain length);
+   a membership that fits is installed by uploading the graph's slot table in
+   stream order, with no capture.  No counterpart in the reference. */
+typedef struct scmf_batch_stats_t {
+  uint64_t steps;            /* steps enqueued */
+  uint64_t captures;         /* step graphs captured */
+  uint64_t table_uploads;    /* memberships installed into a captured graph */
+  int32_t members, member_capacity;
+  uint64_t blocks_launched;  /* workgroups per step, all phases */
+  uint64_t blocks_used;      /* of those, the current members' */
+} scmf_batch_stats_t;
+int scmf_batch_stats(scmf_batch* batch, scmf_batch_stats_t* out);
+
 /* ---- serving: jobs submitted at any time, run by continuous batching -------
  * No counterpart in the reference.  A job is a system (description, options)
  * and a sweep count.  A worker thread admits queued jobs in arrival order
@@ -331,6 +344,8 @@ typedef struct scmf_server_stats_t {
   double span_s;         /* from the start of the first tick to the end of the last
                             finished one; busy_s / span_s is the GPU's busy fraction */
   uint64_t used_bytes;   /* admission estimates of the jobs holding memory */
+  uint64_t captures;     /* step graphs captured; the other membership changes only
+                            uploaded the graph's slot table */
 } scmf_server_stats_t;
 
 typedef struct scmf_server scmf_server;

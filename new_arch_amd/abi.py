@@ -108,7 +108,14 @@ CANCEL_NONE, CANCEL_CALLER, CANCEL_TIMEOUT = range(3)
 class ServerStats(C.Structure):
     _fields_ = [("ticks", C.c_uint64), ("rebuilds", C.c_uint64), ("admitted", C.c_uint64),
                 ("finished", C.c_uint64), ("cancelled", C.c_uint64), ("failed", C.c_uint64),
-                ("busy_s", C.c_double), ("span_s", C.c_double), ("used_bytes", C.c_uint64)]
+                ("busy_s", C.c_double), ("span_s", C.c_double), ("used_bytes", C.c_uint64),
+                ("captures", C.c_uint64)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("captures", C.c_uint64), ("table_uploads", C.c_uint64),
+                ("members", C.c_int32), ("member_capacity", C.c_int32),
+                ("blocks_launched", C.c_uint64), ("blocks_used", C.c_uint64)]
 
 
 class JobInfo(C.Structure):

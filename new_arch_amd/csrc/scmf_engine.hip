@@ -2978,12 +2978,22 @@ int scmf_density_to_vtk(const char* density_path, const char* vtk_path) {
 // read or written through their handles between batch calls (after
 // scmf_batch_sync).  A member may carry a stop: from that sweep count on its
 // workgroups return at once, so members of different lengths share every
-// step.  A change of membership or stops builds the next generation of tables
-// and graph without waiting for the queued steps: the tables go to a
-// stream-ordered allocation filled from pinned staging on the batch stream,
-// the graph is captured on a stream of its own, and the previous generation's
-// memory is freed in stream order, its graph destroyed once an event after
-// its last launch has passed.
+// step.
+//
+// The step graph is static across changes of membership and stops.  Its
+// launches are sized to capacities (members, workgroups per phase, LDS rows
+// per sweep shape) and read everything else from a device-side slot table:
+// the members' parameter records and per-phase block tables.  Entries beyond
+// the current membership name a vacant record (stop 0, sweep words zero), so
+// their workgroups return at once.  A new membership that fits is installed
+// by uploading the table on the batch stream, in order with the queued steps,
+// without waiting for them and without a capture.  One that does not fit (a
+// new sweep shape, longer chains, more members or workgroups, or far fewer,
+// or a member gridded in tiles, whose launches name it) builds the next
+// generation: its table in a stream-ordered allocation, its graph captured on
+// a stream of its own with half again the room needed, the previous
+// generation's memory freed in stream order and its graph destroyed once an
+// event after its last launch has passed.
 struct scmf_batch {
   std::vector<scmf_sys*> m;
   std::vector<uint32_t> stop;    // per member: the sweep count it stops at (NO_STOP: none)
@@ -2991,18 +3001,24 @@ struct scmf_batch {
   hipStream_t cap = nullptr;     // graph capture only
   struct Group {
     const SweepVariant* v;
-    int nmax;
+    int nmax;                    // LDS rows the launch is sized for
     size_t lds;
     size_t bt;                   // byte offset of its block table in d_buf
-    unsigned nblocks;
+    unsigned nblocks;            // workgroups launched (the table's capacity)
   };
-  // the current generation
-  uint8_t* d_buf = nullptr;      // KParams of every member, then the block tables
+  // the current generation: the captured graph and the slot table it reads
+  uint8_t* d_buf = nullptr;      // KParams of member_cap members and the vacant record,
+                                 // the block tables, the vacant record's sweep words
+  size_t buf_bytes = 0;
+  int member_cap = 0;            // the vacant record is Ps[member_cap]
+  bool tiled = false;            // the graph holds per-member tiled gridding launches
   std::vector<Group> groups;
-  size_t bt_zero = 0, bt_grid = 0, bt_omega = 0;
+  size_t bt_zero = 0, bt_grid = 0, bt_omega = 0, vac_ctr = 0;
   unsigned n_zero = 0, n_grid = 0, n_omega = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  uint64_t uploads = 0;          // memberships installed into the current graph's table
+  uint64_t used_blocks = 0;      // workgroups of the current members (of those launched)
   struct Retired {
     hipGraph_t graph;
     hipGraphExec_t exec;
@@ -3067,6 +3083,10 @@ int batch_retire(scmf_batch* b) {
     b->d_buf = nullptr;
   }
   b->groups.clear();
+  b->buf_bytes = 0;
+  b->member_cap = 0;
+  b->tiled = false;
+  b->n_zero = b->n_grid = b->n_omega = 0;
   return SCMF_OK;
 }
 
@@ -3103,36 +3123,24 @@ uint32_t stride_blocks(int64_t work) {
 
 size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 
-// Build the next generation: parameters, block tables and the step graph.
-int batch_build(scmf_batch* b) {
-  batch_reclaim(b, false);
-  int rc = batch_retire(b);
-  if (rc) return rc;
+// What one membership needs: the workgroups of each sweep shape, with the
+// longest chain among its members, and of the clear, gridding and omega
+// phases; members ordered by decreasing beads, so the largest start first.
+struct BatchNeed {
+  std::vector<const SweepVariant*> v;
+  std::vector<int> nmax;
+  std::vector<std::vector<BlockRef>> sweep;
+  std::vector<BlockRef> zero, grid, omega;
+  bool tiled = false;  // a member grids in tiles (its own launches)
+};
+
+BatchNeed batch_need(scmf_batch* b) {
+  BatchNeed nd;
   const int n = (int)b->m.size();
-  if (n == 0) return SCMF_OK;
-  // host image of the generation's device buffer
-  std::vector<uint8_t> blob(align16(sizeof(KParams) * n));
-  {
-    KParams* ps = (KParams*)blob.data();
-    for (int i = 0; i < n; ++i) {
-      ps[i] = b->m[i]->kp;
-      ps[i].cells = nullptr;  // the batch grids from positions
-      ps[i].sched = nullptr;  // and schedules inside the sweep
-      ps[i].stop = b->stop[i];
-    }
-  }
-  auto add_table = [&](const std::vector<BlockRef>& t) {
-    const size_t off = blob.size();
-    blob.resize(align16(off + sizeof(BlockRef) * std::max<size_t>(1, t.size())), 0);
-    if (!t.empty()) memcpy(blob.data() + off, t.data(), sizeof(BlockRef) * t.size());
-    return off;
-  };
-  // sweep groups by kernel shape; members ordered by decreasing work first
   std::vector<int> order(n);
   for (int i = 0; i < n; ++i) order[i] = i;
   std::sort(order.begin(), order.end(),
             [&](int a, int c) { return b->m[a]->n_beads > b->m[c]->n_beads; });
-  std::vector<std::vector<BlockRef>> tabs;
   for (int i : order) {
     scmf_sys* s = b->m[i];
     const SweepVariant* v = s->variant;
@@ -3146,53 +3154,160 @@ int batch_build(scmf_batch* b) {
       s->lds_bytes = keepl;
       s->opts.lanes_per_chain = keep;
     }
-    int gi = -1;
-    for (size_t k = 0; k < b->groups.size(); ++k)
-      if (b->groups[k].v == v) gi = (int)k;
-    if (gi < 0) {
-      b->groups.push_back({v, 0, 0, 0, 0});
-      tabs.emplace_back();
-      gi = (int)b->groups.size() - 1;
+    size_t gi = 0;
+    while (gi < nd.v.size() && nd.v[gi] != v) ++gi;
+    if (gi == nd.v.size()) {
+      nd.v.push_back(v);
+      nd.nmax.push_back(0);
+      nd.sweep.emplace_back();
     }
-    scmf_batch::Group& g = b->groups[gi];
-    g.nmax = std::max(g.nmax, s->nmax);
+    nd.nmax[gi] = std::max(nd.nmax[gi], s->nmax);
     const int G = v->L == 0 ? BLOCK : 64 / v->L;
     const uint32_t nb = (uint32_t)((s->n_chains + G - 1) / G);
-    for (uint32_t k = 0; k < nb; ++k) tabs[gi].push_back({(uint32_t)i, k, nb, 0});
+    for (uint32_t k = 0; k < nb; ++k) nd.sweep[gi].push_back({(uint32_t)i, k, nb, 0});
   }
-  for (size_t k = 0; k < b->groups.size(); ++k) {
-    scmf_batch::Group& g = b->groups[k];
-    g.lds = g.v->bytes(g.nmax);
+  for (int i : order) {
+    scmf_sys* s = b->m[i];
+    const uint32_t nz = stride_blocks(s->count_words);
+    for (uint32_t k = 0; k < nz; ++k) nd.zero.push_back({(uint32_t)i, k, nz, 0});
+    if (s->tile_lds && s->sorted) {
+      nd.tiled = true;
+    } else {
+      const uint32_t ng = stride_blocks(s->n_beads);
+      for (uint32_t k = 0; k < ng; ++k) nd.grid.push_back({(uint32_t)i, k, ng, 0});
+    }
+    const uint32_t no = stride_blocks(s->n_cells);
+    for (uint32_t k = 0; k < no; ++k) nd.omega.push_back({(uint32_t)i, k, no, 0});
+  }
+  return nd;
+}
+
+int batch_group_of(const scmf_batch* b, const SweepVariant* v) {
+  for (size_t k = 0; k < b->groups.size(); ++k)
+    if (b->groups[k].v == v) return (int)k;
+  return -1;
+}
+
+// Whether the captured graph can run this membership from a new table: every
+// sweep shape captured with room for its workgroups and chain length, room in
+// every phase, no tiled launches either side, and at least a quarter of the
+// launched workgroups in use (else a smaller graph pays for its capture).
+bool batch_fits(const scmf_batch* b, const BatchNeed& nd, int n) {
+  if (!b->exec || b->tiled || nd.tiled || n > b->member_cap) return false;
+  uint64_t used = nd.zero.size() + nd.grid.size() + nd.omega.size(), launched = 0;
+  for (size_t k = 0; k < nd.v.size(); ++k) {
+    const int gi = batch_group_of(b, nd.v[k]);
+    if (gi < 0) return false;
+    const scmf_batch::Group& g = b->groups[gi];
+    if (nd.nmax[k] > g.nmax || nd.sweep[k].size() > g.nblocks) return false;
+    used += nd.sweep[k].size();
+  }
+  if (nd.zero.size() > b->n_zero || nd.grid.size() > b->n_grid || nd.omega.size() > b->n_omega)
+    return false;
+  for (const scmf_batch::Group& g : b->groups) launched += g.nblocks;
+  launched += (uint64_t)b->n_zero + b->n_grid + b->n_omega;
+  return launched <= 4096 || 4 * used >= launched;
+}
+
+// Fill the current generation's table for this membership (vacant entries
+// past it) and upload it on the batch stream.
+int batch_install(scmf_batch* b, const BatchNeed& nd) {
+  const int n = (int)b->m.size();
+  std::vector<uint8_t> blob(b->buf_bytes, 0);
+  KParams* ps = (KParams*)blob.data();
+  for (int i = 0; i < n; ++i) {
+    ps[i] = b->m[i]->kp;
+    ps[i].cells = nullptr;  // the batch grids from positions
+    ps[i].sched = nullptr;  // and schedules inside the sweep
+    ps[i].stop = b->stop[i];
+  }
+  KParams vacant{};
+  vacant.stop = 0;  // stopped at sweep word 0, step word 0: every phase returns at once
+  vacant.ctr = (uint32_t*)(b->d_buf + b->vac_ctr);
+  for (int i = n; i <= b->member_cap; ++i) ps[i] = vacant;
+  const BlockRef none{(uint32_t)b->member_cap, 0, 1, 0};
+  auto put = [&](size_t off, unsigned cap, const std::vector<BlockRef>& t) {
+    BlockRef* dst = (BlockRef*)(blob.data() + off);
+    for (unsigned k = 0; k < cap; ++k) dst[k] = k < t.size() ? t[k] : none;
+  };
+  uint64_t used = nd.zero.size() + nd.grid.size() + nd.omega.size();
+  static const std::vector<BlockRef> empty;
+  for (const scmf_batch::Group& g : b->groups) {
+    size_t k = 0;
+    while (k < nd.v.size() && nd.v[k] != g.v) ++k;
+    put(g.bt, g.nblocks, k < nd.v.size() ? nd.sweep[k] : empty);
+    if (k < nd.v.size()) used += nd.sweep[k].size();
+  }
+  put(b->bt_zero, b->n_zero, nd.zero);
+  put(b->bt_grid, b->n_grid, nd.grid);
+  put(b->bt_omega, b->n_omega, nd.omega);
+  b->used_blocks = used;
+  return batch_upload(b, b->d_buf, blob.data(), blob.size());
+}
+
+// Workgroups a capacity holds: the need, with half again as much room when
+// the membership is expected to change.
+unsigned batch_room(size_t need, bool room) {
+  return (unsigned)std::max<size_t>(1, room ? need + need / 2 + 64 : need);
+}
+
+// Install the current membership: into the captured graph's table when it
+// fits, else as a new generation (room: capacities with headroom).
+int batch_build(scmf_batch* b, bool room) {
+  batch_reclaim(b, false);
+  const BatchNeed nd = batch_need(b);
+  const int n = (int)b->m.size();
+  if (batch_fits(b, nd, n)) {
+    int rc = batch_install(b, nd);
+    if (!rc) b->uploads += 1;
+    return rc;
+  }
+  int rc = batch_retire(b);
+  if (rc) return rc;
+  if (n == 0) return SCMF_OK;
+  // the shape of the new generation
+  b->member_cap = room ? n + n / 2 + 4 : n;
+  b->tiled = nd.tiled;
+  for (size_t k = 0; k < nd.v.size(); ++k) {
+    scmf_batch::Group g{nd.v[k], nd.nmax[k], 0, 0, batch_room(nd.sweep[k].size(), room)};
+    // rows rounded up to 16 with room, unless that no longer fits the LDS
+    if (room && nd.v[k]->L != 0) {
+      const int up = (nd.nmax[k] + 15) & ~15;
+      if (up == nd.nmax[k] || nd.v[k]->bytes(up) <= 65536 ||
+          raise_lds_limit(nd.v[k]->fn, nd.v[k]->bytes(up)) == hipSuccess)
+        g.nmax = up;
+      else
+        (void)hipGetLastError();
+    }
+    g.lds = g.v->L == 0 ? 0 : g.v->bytes(g.nmax);
     if (g.lds > 65536 && raise_lds_limit(g.v->fn, g.lds) != hipSuccess) {
       (void)hipGetLastError();
       return fail(SCMF_E_ARG, "batch members' chains are too long for one LDS tile shape");
     }
-    g.nblocks = (unsigned)tabs[k].size();
-    g.bt = add_table(tabs[k]);
+    b->groups.push_back(g);
   }
-  std::vector<BlockRef> tz, tg, to;
-  for (int i : order) {
-    scmf_sys* s = b->m[i];
-    const uint32_t nz = stride_blocks(s->count_words);
-    for (uint32_t k = 0; k < nz; ++k) tz.push_back({(uint32_t)i, k, nz, 0});
-    if (!(s->tile_lds && s->sorted)) {
-      const uint32_t ng = stride_blocks(s->n_beads);
-      for (uint32_t k = 0; k < ng; ++k) tg.push_back({(uint32_t)i, k, ng, 0});
-    }
-    const uint32_t no = stride_blocks(s->n_cells);
-    for (uint32_t k = 0; k < no; ++k) to.push_back({(uint32_t)i, k, no, 0});
+  b->n_zero = batch_room(nd.zero.size(), room);
+  b->n_grid = nd.grid.empty() && !room ? 0 : batch_room(nd.grid.size(), room);
+  b->n_omega = batch_room(nd.omega.size(), room);
+  // layout: parameter records, tables, the vacant record's sweep words
+  size_t off = align16(sizeof(KParams) * (size_t)(b->member_cap + 1));
+  for (scmf_batch::Group& g : b->groups) {
+    g.bt = off;
+    off = align16(off + sizeof(BlockRef) * g.nblocks);
   }
-  b->bt_zero = add_table(tz);
-  b->bt_grid = add_table(tg);
-  b->bt_omega = add_table(to);
-  b->n_zero = (unsigned)tz.size();
-  b->n_grid = (unsigned)tg.size();
-  b->n_omega = (unsigned)to.size();
-  HIP_TRY(hipMallocAsync((void**)&b->d_buf, blob.size(), b->stream));
-  if ((rc = batch_upload(b, b->d_buf, blob.data(), blob.size()))) return rc;
+  b->bt_zero = off;
+  off = align16(off + sizeof(BlockRef) * b->n_zero);
+  b->bt_grid = off;
+  off = align16(off + sizeof(BlockRef) * std::max(1u, b->n_grid));
+  b->bt_omega = off;
+  off = align16(off + sizeof(BlockRef) * b->n_omega);
+  b->vac_ctr = off;
+  b->buf_bytes = off + 16;
+  HIP_TRY(hipMallocAsync((void**)&b->d_buf, b->buf_bytes, b->stream));
+  if ((rc = batch_install(b, nd))) return rc;
   // capture one step on the capture stream
   const KParams* Ps = (const KParams*)b->d_buf;
-  auto table = [&](size_t off) { return (const BlockRef*)(b->d_buf + off); };
+  auto table = [&](size_t o) { return (const BlockRef*)(b->d_buf + o); };
   std::lock_guard<std::mutex> capture_lock(scmf_batch_tickets::capture_mutex());
   HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
   for (const scmf_batch::Group& g : b->groups) {
@@ -3205,7 +3320,8 @@ int batch_build(scmf_batch* b) {
       (void)hipLaunchKernel(g.v->fn, dim3(g.nblocks), dim3(64), args, g.lds, b->cap);
     }
   }
-  hipLaunchKernelGGL(k_advance, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, b->cap, Ps, n);
+  const int nk = b->member_cap + 1;
+  hipLaunchKernelGGL(k_advance, dim3((unsigned)((nk + 63) / 64)), dim3(64), 0, b->cap, Ps, nk);
   hipLaunchKernelGGL(k_zero_counts, dim3(b->n_zero), dim3(BLOCK), 0, b->cap, Ps, table(b->bt_zero));
   if (b->n_grid)
     hipLaunchKernelGGL(k_grid, dim3(b->n_grid), dim3(BLOCK), 0, b->cap, Ps, table(b->bt_grid));
@@ -3268,7 +3384,7 @@ int batch_launch(scmf_batch* b, int32_t n_steps) {
       }
     }
     if (reordered) {
-      int rc = batch_build(b);
+      int rc = batch_build(b, false);
       if (rc) return rc;
     }
     if (!b->exec) break;  // no members
@@ -3363,7 +3479,7 @@ int scmf_batch_create(scmf_batch** out, scmf_sys* const* members, int32_t n_memb
   }
   b->origin_host = std::chrono::steady_clock::now();
   if (!rc) rc = batch_prepare_members(members, n_members);
-  if (!rc) rc = batch_build(b);
+  if (!rc) rc = batch_build(b, false);
   if (rc) {
     scmf_batch_destroy(b);
     return rc;
@@ -3386,7 +3502,7 @@ int scmf_batch_update(scmf_batch* b, scmf_sys* const* members, int32_t n_members
   b->stop.assign(n_members, NO_STOP);
   if (stop_sweeps)
     for (int32_t i = 0; i < n_members; ++i) b->stop[i] = stop_of(stop_sweeps[i]);
-  return batch_build(b);
+  return batch_build(b, true);
 }
 
 void scmf_batch_destroy(scmf_batch* b) {
@@ -3425,7 +3541,22 @@ int scmf_batch_sync(scmf_batch* b) {
 int scmf_batch_set_lanes(scmf_batch* b, int32_t lanes_per_chain) {
   if (!b) return fail(SCMF_E_ARG, "null batch");
   b->forced_L = lanes_per_chain;
-  return batch_build(b);
+  return batch_build(b, false);
+}
+
+int scmf_batch_stats(scmf_batch* b, scmf_batch_stats_t* out) {
+  if (!b || !out) return fail(SCMF_E_ARG, "null argument");
+  memset(out, 0, sizeof *out);
+  out->steps = b->steps;
+  out->captures = b->generations;
+  out->table_uploads = b->uploads;
+  out->members = (int32_t)b->m.size();
+  out->member_capacity = b->member_cap;
+  uint64_t launched = (uint64_t)b->n_zero + b->n_grid + b->n_omega;
+  for (const scmf_batch::Group& g : b->groups) launched += g.nblocks;
+  out->blocks_launched = b->exec ? launched : 0;
+  out->blocks_used = b->exec ? b->used_blocks : 0;
+  return SCMF_OK;
 }
 
 }  // extern "C"

@@ -124,6 +124,7 @@ struct scmf_server {
   bool dirty = false;
   uint64_t next_id = 1;
   uint64_t ticks = 0, rebuilds = 0;
+  uint64_t captures = 0;                       // step graphs the batch captured (under mu)
   double busy_s = 0, first_start = -1, last_end = 0;
 
   double now() const {
@@ -417,6 +418,11 @@ int scmf_server_tick(scmf_server* sv, int32_t* n_queued, int32_t* n_running) {
     if (rc) return sfail(rc, std::string("batch: ") + scmf_last_error());
     sv->dirty = false;
     sv->rebuilds += 1;
+    scmf_batch_stats_t bs;
+    if (scmf_batch_stats(sv->batch, &bs) == SCMF_OK) {
+      std::lock_guard<std::mutex> lk(sv->mu);
+      sv->captures = bs.captures;
+    }
   }
   // one tick of the whole batch
   bool enqueued = false;
@@ -559,6 +565,7 @@ int scmf_server_stats(scmf_server* sv, scmf_server_stats_t* out) {
   out->busy_s = sv->busy_s;
   out->span_s = sv->first_start >= 0 ? sv->last_end - sv->first_start : 0.0;
   out->used_bytes = sv->used;
+  out->captures = sv->captures;
   return SCMF_OK;
 }
 

@@ -8,8 +8,8 @@ import os
 
 import numpy as np
 
-from .abi import (DENSITY, OMEGA, POS, SCMF_OK, Desc, JobInfo, Opts, ServerOpts, ServerStats,
-                  Stats, Summary, SystemSpec, Timing, make_opts)
+from .abi import (DENSITY, OMEGA, POS, SCMF_OK, BatchStats, Desc, JobInfo, Opts, ServerOpts,
+                  ServerStats, Stats, Summary, SystemSpec, Timing, make_opts)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libscmf.so")
@@ -73,6 +73,7 @@ def lib():
     L.scmf_batch_set_lanes.argtypes = [vp, C.c_int32]
     L.scmf_batch_update.argtypes = [vp, C.POINTER(vp), C.c_int32, vp]
     L.scmf_batch_destroy.argtypes = [vp]
+    L.scmf_batch_stats.argtypes = [vp, C.POINTER(BatchStats)]
     L.scmf_batch_destroy.restype = None
     L.scmf_server_create.argtypes = [C.POINTER(vp), C.POINTER(ServerOpts)]
     L.scmf_server_submit.argtypes = [vp, C.POINTER(Desc), C.POINTER(Opts), C.c_int32,
@@ -313,6 +314,11 @@ class Batch:
             st = np.ascontiguousarray([-1 if v is None else int(v) for v in stops], np.int64)
         _check(lib().scmf_batch_update(self._h, arr, len(self.members),
                                        st.ctypes.data if st is not None else None))
+
+    def stats(self) -> BatchStats:
+        st = BatchStats()
+        _check(lib().scmf_batch_stats(self._h, C.byref(st)))
+        return st
 
     def close(self):
         if self._h:

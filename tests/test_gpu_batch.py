@@ -24,13 +24,13 @@ def mix():
     return systems
 
 
-def make(systems, lanes=0):
+def make(systems, lanes=0, sort_every=3):
     out = []
     for k, (spec, seed, sid) in enumerate(systems):
         out.append(scmf.System(spec, make_opts(seed=seed, system_id=sid,
                                                omega_every_step=(k % 2 == 0),
                                                density_bits=16 if k % 3 else 32,
-                                               sort_every=3, lanes_per_chain=lanes)))
+                                               sort_every=sort_every, lanes_per_chain=lanes)))
     for g in out:
         g.grid_density()
         g.update_omega()
@@ -108,6 +108,58 @@ def test_stops_and_membership_changes_while_steps_are_queued():
         np.testing.assert_array_equal(a.density(), m.density(), err_msg=f"member {i} density")
         assert same(a.omega(), m.omega()), f"member {i} omega"
         assert a.stats().accepts == m.stats().accepts
+    b.close()
+    for g in alone + mem:
+        g.close()
+
+
+def test_membership_changes_reuse_the_captured_graph():
+    """A membership that fits the captured graph's room is installed by a
+    table upload alone: members join and leave, the graph is captured only
+    when the room runs out, vacant slots do no work, and every member ends
+    where its own steps alone leave it."""
+    banner("static graph across membership changes")
+    systems = W.batched_mix(n_systems=8, seed=21, sizes=(16, 24), lengths=(16, 24, 32))
+    mem = make(systems, sort_every=0)          # atomic gridding: no per-member launches
+    stops = [6, 10, 4, 12, 8, 9, 5, 7]
+    b = scmf.Batch(mem[:2])
+    b.set_lanes(8)
+    st0 = b.stats()
+    assert st0.captures >= 1 and st0.members == 2
+    b.update(mem[:3], stops[:3])              # grows past the exact first graph: captured with room
+    st1 = b.stats()
+    assert st1.captures == st0.captures + 1
+    assert st1.member_capacity >= 4 and st1.blocks_used <= st1.blocks_launched
+    done = [0] * len(mem)
+
+    def advance(members, n):
+        for i in members:
+            done[i] = min(stops[i], done[i] + n)
+
+    b.step(3)
+    advance([0, 1, 2], 3)
+    plan = [[0, 1, 2, 3], [1, 2, 3], [1, 3, 4], [1, 3, 4, 5], [4, 5]]
+    for members in plan:
+        b.update([mem[i] for i in members], [stops[i] for i in members])
+        b.step(2)
+        advance(members, 2)
+    st2 = b.stats()
+    assert st2.captures == st1.captures, "a membership inside the room captured a graph"
+    assert st2.table_uploads >= st1.table_uploads + len(plan)
+    b.step(12)
+    advance(plan[-1], 12)
+    b.sync()
+    alone = make(systems, sort_every=0)
+    for i, g in enumerate(alone):
+        for k in range(1, done[i] + 1):
+            g.step(k)
+    for i, (a, m) in enumerate(zip(alone, mem)):
+        assert m.stats().sweeps == done[i], (i, m.stats().sweeps, done[i])
+        assert same(a.positions(), m.positions()), f"member {i} positions"
+        np.testing.assert_array_equal(a.density(), m.density(), err_msg=f"member {i} density")
+        assert same(a.omega(), m.omega()), f"member {i} omega"
+        assert a.stats().accepts == m.stats().accepts
+    assert done[6] == done[7] == 0 and done[0] < stops[0]   # 6, 7 never ran; 0 left early
     b.close()
     for g in alone + mem:
         g.close()
