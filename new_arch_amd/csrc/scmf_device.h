@@ -51,6 +51,8 @@ struct KParams {
   int linear[MAX_P];             // 1: bonds are exactly i <-> i+1 (neighbours computed, not looked up)
   double inv_len[MAX_P];         // 1/len, for the bead -> (chain, monomer) split
   int mono_off[MAX_P + 1];
+  int type_bits;                 // 1: at most two monomer types and 64 beads a chain, so
+  uint64_t tbits[MAX_P];         //    bead b of architecture p has type (tbits[p] >> b) & 1
   const int* monotype;           // [sum N_p]
   const int* nbr_start;          // [sum N_p + 1], CSR over every bead of every type
   const int* nbr;                // neighbour bead indices within the chain

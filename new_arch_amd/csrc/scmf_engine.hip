@@ -160,14 +160,65 @@ template <class KP, class V>
 struct Neighbours<false, KP, V> {
   const KP& kp;
   const V& c;
-  int p, b, len;
-  __device__ Neighbours(const KP& kp_, const V& c_, int p_, int b_, int len_)
-      : kp(kp_), c(c_), p(p_), b(b_), len(len_) {}
+  int moff, b;
+  __device__ Neighbours(const KP& kp_, const V& c_, int moff_, int b_, int)
+      : kp(kp_), c(c_), moff(moff_), b(b_) {}
   template <class F>
   __device__ void each(F&& f) const {
-    for_neighbours(kp, p, b, len, [&](int j) { f(c.X(j), c.Y(j), c.Z(j)); });
+    const SCMF_GLOBAL int* ns = gptr((const int*)kp.nbr_start);
+    const SCMF_GLOBAL int* nb = gptr((const int*)kp.nbr);
+    const int k0 = ns[moff + b], k1 = ns[moff + b + 1];
+    for (int k = k0; k < k1; ++k) {
+      const int j = nb[k];
+      f(c.X(j), c.Y(j), c.Z(j));
+    }
   }
 };
+
+// What the moves of one chain read about its architecture, loaded once per
+// chain before its moves (the loads would otherwise open every move, one
+// waiting on the next): the chain length, whether its bonds are i <-> i+1,
+// where its types and neighbour lists start, and its types as a bit mask
+// when the system has at most two types and 64 beads a chain.
+struct ChainArch {
+  int length, moff;
+  bool linear;
+  uint64_t tbits;
+};
+
+template <class KP>
+__device__ __forceinline__ ChainArch chain_arch(const KP& kp, int p) {
+  ChainArch a;
+  a.length = kp.len[p];
+  a.moff = kp.mono_off[p];
+  a.linear = kp.linear[p] != 0;
+  a.tbits = kp.tbits[p];
+  return a;
+}
+
+// The monomer type of bead b (monotype, init_config.cu:338).
+template <class KP>
+__device__ __forceinline__ uint32_t bead_type(const KP& kp, const ChainArch& a, int b) {
+  if (kp.type_bits) return (uint32_t)(a.tbits >> b) & 1u;
+  return (uint32_t)gptr((const int*)kp.monotype)[a.moff + b];
+}
+
+// A[t] and R[t] chosen among the system's values (uniform, in scalar
+// registers) instead of loaded by a per-lane index.
+template <class KP>
+__device__ __forceinline__ void smart_step(const KP& kp, uint32_t t, float* A, float* R) {
+  float a = kp.A[0], r = kp.R[0];
+#pragma unroll
+  for (int k = 1; k < MAX_T; ++k) {
+    if (k >= kp.T) break;
+    if (t == (uint32_t)k) {
+      a = kp.A[k];
+      r = kp.R[k];
+    }
+  }
+  *A = a;
+  *R = r;
+}
 
 // add_bond_forces_arr, device_function.cu:589-634: neighbour difference in
 // fp32, scaled and accumulated in fp64, added to the fp32 force.  (d * 2.0)
@@ -192,21 +243,21 @@ __device__ __forceinline__ void bond_force(const N& nb, double harm2, float px, 
 // Move i of the sweep on bead ibead (its pick, drawn by the caller from the
 // move's SLOT_PICK block).  Returns 1 if accepted.
 template <bool LINEAR, class KP, class V>
-__device__ __forceinline__ uint32_t one_move_n(const KP& kp, const V& c, int p, int length,
+__device__ __forceinline__ uint32_t one_move_n(const KP& kp, const V& c, const ChainArch& a,
                                                uint32_t chain, uint32_t sweep, uint32_t i,
                                                int ibead,
                                                const SCMF_GLOBAL float* __restrict__ omega) {
-  const uint32_t iwtype = (uint32_t)gptr((const int*)kp.monotype)[kp.mono_off[p] + ibead];
+  const uint32_t iwtype = bead_type(kp, a, ibead);
   const u32x4 disp = stream_block(kp.seed, kp.system_id, chain, sweep, i, SLOT_DISP);
   const float ox = c.X(ibead), oy = c.Y(ibead), oz = c.Z(ibead);
-  const Neighbours<LINEAR, KP, V> nb(kp, c, p, ibead, length);
+  const Neighbours<LINEAR, KP, V> nb(kp, c, LINEAR ? 0 : a.moff, ibead, a.length);
   const double harm2 = kp.harm * 2.0;
   float dx, dy, dz;
   double smc_deltaE = 0.0;
   if (kp.move_type == SCMF_MOVE_SMART) {
     float px = ox, py = oy, pz = oz;
-    const float A = kp.A[iwtype];
-    const float R = kp.R[iwtype];
+    float A, R;
+    smart_step(kp, iwtype, &A, &R);
     float fx = 0.0f, fy = 0.0f, fz = 0.0f;
     bond_force(nb, harm2, px, py, pz, &fx, &fy, &fz);
     float rx, ry, rz;
@@ -265,11 +316,11 @@ __device__ __forceinline__ uint32_t one_move_n(const KP& kp, const V& c, int p, 
 
 // The move with neighbour storage sized for the chain's architecture.
 template <class KP, class V>
-__device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, int p, int length,
+__device__ __forceinline__ uint32_t one_move(const KP& kp, const V& c, const ChainArch& a,
                                              uint32_t chain, uint32_t sweep, uint32_t i, int ibead,
                                              const SCMF_GLOBAL float* __restrict__ omega) {
-  if (kp.linear[p]) return one_move_n<true>(kp, c, p, length, chain, sweep, i, ibead, omega);
-  return one_move_n<false>(kp, c, p, length, chain, sweep, i, ibead, omega);
+  if (a.linear) return one_move_n<true>(kp, c, a, chain, sweep, i, ibead, omega);
+  return one_move_n<false>(kp, c, a, chain, sweep, i, ibead, omega);
 }
 
 // The N moves of one chain in order, on one lane: one pick block per two moves.
@@ -277,12 +328,13 @@ template <class KP, class V>
 __device__ __forceinline__ uint32_t chain_moves(const KP& kp, const V& c, int p, int length,
                                                 uint32_t chain, uint32_t sweep,
                                                 const SCMF_GLOBAL float* __restrict__ omega) {
+  const ChainArch a = chain_arch(kp, p);
   uint32_t acc = 0;
   u32x4 pk{};
   for (int i = 0; i < length; i++) {
     if ((i & 1) == 0) pk = stream_block(kp.seed, kp.system_id, chain, sweep, (uint32_t)i >> 1, SLOT_PICK);
     const int ib = pick_bead(pick_u01(pk, (uint32_t)i), length);
-    acc += one_move(kp, c, p, length, chain, sweep, (uint32_t)i, ib, omega);
+    acc += one_move(kp, c, a, chain, sweep, (uint32_t)i, ib, omega);
   }
   return acc;
 }
@@ -515,6 +567,7 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
   }
   uint32_t acc = 0;
   const LdsChain<G> c{sx + slot, sy + slot, sz + slot};
+  const ChainArch ca = chain_arch(kp, p);
   int l = 1, j = 0;
   while (__any(l <= maxl)) {
     if (l <= maxl) {
@@ -532,7 +585,7 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
         } else {
           ib = (int)lev[i];
         }
-        acc += one_move(kp, c, p, length, id, sweep, (uint32_t)i, ib, omega);
+        acc += one_move(kp, c, ca, id, sweep, (uint32_t)i, ib, omega);
       }
       if (++j * L >= cl) {
         ++l;
@@ -1419,6 +1472,16 @@ int upload_arch(scmf_sys* s) {
     }
     s->kp.linear[p] = linear ? 1 : 0;
     coff += (size_t)N * N;
+  }
+  // types as register masks when every chain fits 64 bits of one-bit types
+  bool bits = d.T <= 2;
+  for (size_t p = 0; p < d.poly_length.size(); ++p) bits = bits && d.poly_length[p] <= 64;
+  s->kp.type_bits = bits ? 1 : 0;
+  for (size_t p = 0, m = 0; p < d.poly_length.size(); ++p) {
+    uint64_t tb = 0;
+    for (int b = 0; b < d.poly_length[p]; ++b, ++m)
+      if (bits && d.monotype[m] != 0) tb |= (uint64_t)1 << b;
+    s->kp.tbits[p] = tb;
   }
   nbr_start.push_back((int)nbr.size());
   if (nbr.empty()) nbr.push_back(0);

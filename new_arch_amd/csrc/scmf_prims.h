@@ -325,6 +325,22 @@ SCMF_HD void normal3f(const u32x4& w, float* n0, float* n1, float* n2) {
 }
 
 // ---------------------------------------------------------------- exp
+// fma(p, r, c) for a constant c.  On the GPU the addend is read from a scalar
+// register pair (one v_fma_f64); left to itself the compiler keeps every
+// fp64 coefficient of a polynomial in a vector register pair, hoisted out of
+// the enclosing loop, and copies it before a two-operand fma, which costs a
+// move per coefficient and spills in a kernel short of registers.  The
+// arithmetic is the same single fused multiply-add.
+SCMF_HD double fma_c(double p, double r, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double out;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(out) : "v"(p), "v"(r), "s"(c));
+  return out;
+#else
+  return __builtin_fma(p, r, c);
+#endif
+}
+
 // exp(x) in double.  k = nearest integer to x/ln2; r = x - k ln2 in two steps
 // (Cody & Waite 1980), |r| <= 0.347; exp(r) by its Taylor series to r^14
 // (truncation < 2e-18 relative); scale by 2^k in two halves so gradual
@@ -338,17 +354,17 @@ SCMF_HD double exp_cw(double x) {
   const double LN2_LO = 0x1.f473de6af278fp-30;
   const double r = __builtin_fma(-kd, LN2_LO, __builtin_fma(-kd, LN2_HI, x));
   double p = 1.0 / 87178291200.0;  // 1/14!
-  p = __builtin_fma(p, r, 1.0 / 6227020800.0);
-  p = __builtin_fma(p, r, 1.0 / 479001600.0);
-  p = __builtin_fma(p, r, 1.0 / 39916800.0);
-  p = __builtin_fma(p, r, 1.0 / 3628800.0);
-  p = __builtin_fma(p, r, 1.0 / 362880.0);
-  p = __builtin_fma(p, r, 1.0 / 40320.0);
-  p = __builtin_fma(p, r, 1.0 / 5040.0);
-  p = __builtin_fma(p, r, 1.0 / 720.0);
-  p = __builtin_fma(p, r, 1.0 / 120.0);
-  p = __builtin_fma(p, r, 1.0 / 24.0);
-  p = __builtin_fma(p, r, 1.0 / 6.0);
+  p = fma_c(p, r, 1.0 / 6227020800.0);
+  p = fma_c(p, r, 1.0 / 479001600.0);
+  p = fma_c(p, r, 1.0 / 39916800.0);
+  p = fma_c(p, r, 1.0 / 3628800.0);
+  p = fma_c(p, r, 1.0 / 362880.0);
+  p = fma_c(p, r, 1.0 / 40320.0);
+  p = fma_c(p, r, 1.0 / 5040.0);
+  p = fma_c(p, r, 1.0 / 720.0);
+  p = fma_c(p, r, 1.0 / 120.0);
+  p = fma_c(p, r, 1.0 / 24.0);
+  p = fma_c(p, r, 1.0 / 6.0);
   p = __builtin_fma(p, r, 0.5);
   p = __builtin_fma(p, r, 1.0);
   const double er = __builtin_fma(p, r, 1.0);
