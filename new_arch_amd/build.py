@@ -1,5 +1,7 @@
-"""Build recipe for the native parts: libscmf.so + scmf_run (HIP, gfx950), and
-libscmf_calib.so (kernels of known VALU instruction count, for measurement).
+"""Build recipe for the native parts: libscmf.so + scmf_run (HIP, gfx950),
+libscmf_calib.so (kernels of known VALU instruction count, for measurement)
+and libscmf_check.so (the device builds of the shared arithmetic primitives
+over given inputs, for the tests).
 
 Every HIP translation unit is compiled with FP contraction off (results must
 match the CPU oracle bit for bit) and with packed-FP32 instructions disabled.
@@ -16,6 +18,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libscmf.so")
 RUNNER = os.path.join(LIBDIR, "scmf_run")
 CALIB = os.path.join(LIBDIR, "libscmf_calib.so")  # VALU-rate calibration kernels (measurement)
+CHECK = os.path.join(LIBDIR, "libscmf_check.so")  # device builds of the primitives (tests)
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -43,6 +46,10 @@ def build(verbose=True):
     _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, "-shared", "-o", tmp,
           os.path.join(CSRC, "scmf_calib.hip")])
     os.replace(tmp, CALIB)
+    tmp = CHECK + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", *COMMON, "-shared", "-o", tmp,
+          os.path.join(CSRC, "scmf_check.hip")])
+    os.replace(tmp, CHECK)
     tmp = RUNNER + ".tmp"
     _run([HIPCC, *COMMON, "-o", tmp, os.path.join(CSRC, "scmf_main.cpp"),
           f"-L{LIBDIR}", "-lscmf", "-Wl,-rpath,$ORIGIN"])

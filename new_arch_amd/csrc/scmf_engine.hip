@@ -289,13 +289,15 @@ __device__ __forceinline__ uint32_t one_move_n(const KP& kp, const V& c, const C
   const double energy_new = omega[iwtype * kp.n_cells + cell_new];
   double bonded = 0;
   {
+    // ox - qx is exactly -(qx - ox), the difference the old force used (the
+    // sign of a zero difference cannot reach the squares)
     const double ddx = dx, ddy = dy, ddz = dz;
     nb.each([&](float qx, float qy, float qz) {
-      const double old_rx = (double)(ox - qx);
+      const double old_rx = -(double)(qx - ox);
       const double new_rx = old_rx + ddx;
-      const double old_ry = (double)(oy - qy);
+      const double old_ry = -(double)(qy - oy);
       const double new_ry = old_ry + ddy;
-      const double old_rz = (double)(oz - qz);
+      const double old_rz = -(double)(qz - oz);
       const double new_rz = old_rz + ddz;
       const double old_r2 = old_rx * old_rx + old_ry * old_ry + old_rz * old_rz;
       const double new_r2 = new_rx * new_rx + new_ry * new_ry + new_rz * new_rz;
@@ -481,12 +483,13 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
   // only; otherwise the chain's lanes make it here from lev and W.
   const SCMF_GLOBAL uint8_t* __restrict__ sched = gptr((const uint8_t*)kp.sched);
   const bool pre = sched != nullptr;
-  const int per = pre ? 2 * nmax + 4 : 5 * nmax + 4;
+  const int per = pre ? 3 * nmax + 4 : 5 * nmax + 4;
   I* lev = book + slot * per;
   I* W = lev + nmax;
   I* start = pre ? lev : W + nmax + 2;
   I* list = start + nmax + 2;
-  I* rnd = list + nmax;  // the round of each move (in-sweep scan only)
+  I* rnd = list + nmax;   // the round of each move (in-sweep scan only)
+  I* bead = list + nmax;  // the bead of each move (schedules from the schedule kernels)
   const int64_t chain = (int64_t)br.local * G + slot;
   const bool active = chain < kp.n_chains;
   int p = 0, length = 0;
@@ -510,7 +513,10 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
       const int st = kp.sched_stride;
       maxl = sched[sched_index(chain, st, 0)];
       for (int k = r; k <= length + 1; k += L) start[k] = (I)sched[sched_index(chain, st, 1 + k)];
-      for (int k = r; k < length; k += L) list[k] = (I)sched[sched_index(chain, st, length + 3 + k)];
+      for (int k = r; k < length; k += L) {
+        list[k] = (I)sched[sched_index(chain, st, length + 3 + k)];
+        bead[k] = (I)sched[sched_index(chain, st, 2 * length + 3 + k)];
+      }
     }
     __syncthreads();
   } else {
@@ -575,16 +581,9 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
       const int k = j * L + r;
       if (k < cl) {
         const int i = list[s0 + k];
-        // the pick block's arithmetic at the head of the move measured faster
-        // than loading the bead from a longer schedule record (7.59 against
-        // 6.78 ms a sweep at 268M beads): it overlaps the latency of the loads
-        int ib;
-        if (pre) {
-          const u32x4 pb = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i >> 1, SLOT_PICK);
-          ib = pick_bead(pick_u01(pb, (uint32_t)i), length);
-        } else {
-          ib = (int)lev[i];
-        }
+        // the bead the schedule kernel picked for this move (no pick block
+        // drawn here)
+        const int ib = pre ? (int)bead[i] : (int)lev[i];
         acc += one_move(kp, c, ca, id, sweep, (uint32_t)i, ib, omega);
       }
       if (++j * L >= cl) {
@@ -628,7 +627,7 @@ size_t sched_lds_bytes(int nmax) {
 // The same kernel reading a schedule made by the schedule kernels.
 size_t sched_pre_lds_bytes(int L, int nmax) {
   const int G = 64 / L;
-  return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (2 * nmax + 4);
+  return (size_t)3 * nmax * G * sizeof(float) + (size_t)G * (3 * nmax + 4);
 }
 
 // The move schedule of every chain for the coming sweep, one lane per slot
@@ -638,7 +637,8 @@ size_t sched_pre_lds_bytes(int L, int nmax) {
 // wait for positions.  Record of a slot (uint8, chains of at most 255
 // beads): entry 0 the number of rounds, entries 1 .. N+2 start[0 .. N+1]
 // (the first list entry of each round; zero after the last round), entries
-// N+3 .. 2N+2 the move indices by round.  Records are interleaved by groups
+// N+3 .. 2N+2 the move indices by round, entries 2N+3 .. 3N+2 the bead each
+// move picked (by move index; the sweep then draws no pick block).  Records are interleaved by groups
 // of 8 slots (the chains of one sweep wavefront at 8 lanes per chain): entry
 // k of slot s is byte ((s / 8) * stride + k) * 8 + s % 8, so a sweep
 // wavefront reads its chains' records with contiguous loads.  The lane
@@ -647,14 +647,14 @@ size_t sched_pre_lds_bytes(int L, int nmax) {
 // so LDS per chain (waves per CU) decides its speed.
 constexpr int SCHED_LANES = 64;
 size_t schedule_record_bytes(int nmax, int64_t chains) {
-  return (size_t)(2 * nmax + 3) * SCHED_LANES * (size_t)((chains + SCHED_LANES - 1) / SCHED_LANES);
+  return (size_t)(3 * nmax + 3) * SCHED_LANES * (size_t)((chains + SCHED_LANES - 1) / SCHED_LANES);
 }
 // Whether the register-count kernel takes chains of nmax beads in rounds of cap moves.
 bool schedule_in_registers(int nmax, int cap) {
   return nmax <= 64 && (cap == 2 || cap == 4 || cap == 8);
 }
 size_t schedule_lds_bytes(int nmax, int cap) {
-  return (size_t)(schedule_in_registers(nmax, cap) ? 2 * nmax + 2 : 3 * nmax + 5) * SCHED_LANES;
+  return (size_t)(schedule_in_registers(nmax, cap) ? 2 * nmax + 2 : 4 * nmax + 5) * SCHED_LANES;
 }
 
 // Chains of at most 64 beads, rounds of at most cap = 2^CAPLOG moves.  The
@@ -665,8 +665,9 @@ size_t schedule_lds_bytes(int nmax, int cap) {
 // waits for LDS once per move (its neighbours' rounds).  The rounds used
 // are always 1 .. max, at most one per move, so 64 bits hold them.  LDS
 // holds the round of each bead and of each move (2N + 2 bytes a chain); the
-// record goes to memory byte by byte (staging it in LDS for 16-byte stores
-// costs waves, and measured slower).
+// record goes to memory byte by byte, the bead of each move as the scan
+// picks it (staging the record in LDS for 16-byte stores costs waves, and
+// measured slower).
 template <int CAPLOG>
 __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __restrict__ Ps,
                                                               int nmax) {
@@ -718,6 +719,7 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
     }
     at(W, cur + 1) = (uint8_t)l;
     at(lev, i) = (uint8_t)l;
+    rec[sched_index(chain, stride, 2 * length + 3 + i)] = (uint8_t)cur;
     maxl = max(maxl, l);
   }
   // round starts (start[0] = start[1] = 0, prefix sums through the last
@@ -748,7 +750,8 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
 // in that layout and stored with 16-byte stores (the buffer holds whole
 // runs).  During the scan the round of each move waits in the record's list
 // entries, which the list later fills, and the round of each bead in a lane
-// array, which then takes the moves' rounds: 3N + 5 bytes a chain.
+// array, which then takes the moves' rounds; the bead of each move goes to
+// the record as the scan picks it: 4N + 5 bytes a chain.
 __global__ void __launch_bounds__(SCHED_LANES) k_schedule_lds(const KParams* __restrict__ Ps,
                                                               int nmax, int cap) {
   extern __shared__ __align__(16) uint8_t sh[];
@@ -767,6 +770,7 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_lds(const KParams* __r
   auto at = [](uint8_t* a, int k) -> uint8_t& { return a[k * SCHED_LANES]; };
   auto st = [&](int k) -> uint8_t& { return rl[8 * (1 + k)]; };         // start[k], k <= N + 1
   auto lv = [&](int i) -> uint8_t& { return rl[8 * (length + 3 + i)]; };  // list entry i
+  auto pk = [&](int i) -> uint8_t& { return rl[8 * (2 * length + 3 + i)]; };  // bead of move i
   u32x4 blk{};
   auto pick = [&](int i) {  // one pick block per two moves
     if ((i & 1) == 0) blk = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i >> 1, SLOT_PICK);
@@ -797,6 +801,7 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_lds(const KParams* __r
     st(l + 1) = (uint8_t)(st(l + 1) + 1);
     at(W, cur + 1) = (uint8_t)l;
     lv(i) = (uint8_t)l;  // (the round of move i, until the list is made)
+    pk(i) = (uint8_t)cur;
     maxl = max(maxl, l);
   }
   for (int l = 1; l <= maxl; ++l) st(l + 1) = (uint8_t)(st(l + 1) + st(l));
@@ -1601,6 +1606,7 @@ int launch_schedule(scmf_sys* s, int cap) {
     void* args[] = {(void*)&kp, (void*)&s->nmax};
     HIP_TRY(hipLaunchKernel(fn, grid, dim3(SCHED_LANES), args, lds, s->stream));
   } else {
+    if (lds > 65536) HIP_TRY(raise_lds_limit((const void*)k_schedule_lds, lds));
     void* args[] = {(void*)&kp, (void*)&s->nmax, (void*)&cap};
     HIP_TRY(hipLaunchKernel((const void*)k_schedule_lds, grid, dim3(SCHED_LANES), args, lds,
                             s->stream));
@@ -1970,7 +1976,7 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
       (rc = alloc((void**)&s->d_ctr, 2 * sizeof(uint32_t))))
     return bail(rc);
   if (s->pre_sched) {
-    s->kp.sched_stride = 2 * s->nmax + 3;
+    s->kp.sched_stride = 3 * s->nmax + 3;
     if ((rc = alloc((void**)&s->sched, schedule_record_bytes(s->nmax, s->n_chains))))
       return bail(rc);
     s->kp.sched = s->sched;

@@ -306,17 +306,37 @@ SCMF_HD void sincosf_turn24(uint32_t k, float* s_out, float* c_out) {
   *c_out = c;
 }
 
+// Correctly rounded sqrt of x = -2 log(u), u a 24-bit uniform in (0, 1]: x
+// is +-0 or at least 1.19e-7, never subnormal, infinite or NaN.  On the GPU:
+// v_sqrt_f32 (within 1 ulp) and the two fma residual checks of its
+// neighbours that the compiler's IEEE expansion applies, without that
+// expansion's scaling of tiny arguments and its special-value check, which
+// such an x never needs (+-0 comes back as +-0 through the checks).
+SCMF_HD float sqrtf_bm(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float y = __builtin_amdgcn_sqrtf(x);
+  const float ym = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, y) - 1u);
+  const float yp = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, y) + 1u);
+  const float rm = __builtin_fmaf(-ym, y, x);
+  const float rp = __builtin_fmaf(-yp, y, x);
+  const float r = rm <= 0.0f ? ym : y;
+  return rp > 0.0f ? yp : r;
+#else
+  return __builtin_sqrtf(x);
+#endif
+}
+
 // Three standard normals in fp32 from one Philox block (Box–Muller on
 // 24-bit uniforms): (w0, w1) -> a pair, (w2, w3) -> the cosine branch.
 // This is the displacement noise of a smart-MC move (the reference draws it
 // with curand_normal4 in fp32, device_function.cu:683).
 SCMF_HD void normal3f(const u32x4& w, float* n0, float* n1, float* n2) {
   const float u0 = u01_24f(w.v[0]);  // (0, 1]
-  const float r0 = __builtin_sqrtf(-2.0f * logf_pos(u0));
+  const float r0 = sqrtf_bm(-2.0f * logf_pos(u0));
   float s0, c0;
   sincosf_turn24(w.v[1] >> 8, &s0, &c0);
   const float u2 = u01_24f(w.v[2]);
-  const float r1 = __builtin_sqrtf(-2.0f * logf_pos(u2));
+  const float r1 = sqrtf_bm(-2.0f * logf_pos(u2));
   float s1, c1;
   sincosf_turn24(w.v[3] >> 8, &s1, &c1);
   *n0 = r0 * c0;

@@ -64,6 +64,8 @@ def lib():
     L.orc_sincosf_turn24_many.restype = None
     L.orc_normals_f.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int64, vp]
     L.orc_normals_f.restype = None
+    L.orc_normal3f_words.argtypes = [vp, C.c_int64, vp]
+    L.orc_normal3f_words.restype = None
     L.orc_threads.restype = C.c_int
     _lib = L
     return L
@@ -195,6 +197,14 @@ def sincosf_turn24_many(k):
 def normals_f(seed, system, chain, sweep, n):
     out = np.empty((n, 3), np.float32)
     lib().orc_normals_f(int(seed), int(system), int(chain), int(sweep), int(n), _ptr(out))
+    return out
+
+
+def normal3f_words(words):
+    """normal3f of each 4-word Philox block in `words` (uint32, shape [n, 4])."""
+    w = np.ascontiguousarray(words, np.uint32)
+    out = np.empty((w.shape[0], 3), np.float32)
+    lib().orc_normal3f_words(_ptr(w), w.shape[0], _ptr(out))
     return out
 
 

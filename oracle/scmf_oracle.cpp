@@ -556,6 +556,16 @@ void orc_normals_f(uint64_t seed, uint32_t system, uint32_t chain, uint32_t swee
     normal3f(w, &out[3 * i], &out[3 * i + 1], &out[3 * i + 2]);
   }
 }
+// normal3f of caller-given Philox blocks (4 words each), the host build of
+// the device primitive.
+void orc_normal3f_words(const uint32_t* w, int64_t n, float* out) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    u32x4 b;
+    for (int k = 0; k < 4; ++k) b.v[k] = w[4 * i + k];
+    normal3f(b, &out[3 * i], &out[3 * i + 1], &out[3 * i + 2]);
+  }
+}
 int orc_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();
