@@ -526,7 +526,7 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
     # bytes the implementation moves beyond them: the bead cells the sweep stores
     # for the next gridding, the move-schedule records written and read back
     impl = {"cells_written": 4 * B if from_cells else 0,
-            "schedule_records": 2 * (2 * nmax + 3) * spec.n_chains if pre else 0}
+            "schedule_records": 2 * (3 * nmax + 3) * spec.n_chains if pre else 0}
     grid_bytes_cells = 4 * B + s * TC   # the pass as run: cells read, counts written
     omega_bytes = s * TC + 4 * TC       # counts read, fp32 omega written
     tiled = spec.n_beads >= (1 << 23)
