@@ -536,9 +536,9 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
     # of this workload (calibration and commands in the profile summary).
     measured = profile_json(workload, "traffic").get("bytes_per_launch", {})
     byte_t = "unsigned char" if nmax <= 255 else "unsigned short"
-    # the schedule kernel keeps round counts in registers for chains of <= 64 beads
-    sched_kernel = (f"k_schedule_reg<{ {2: 1, 4: 2, 8: 3}[lanes]}>"
-                    if nmax <= 64 and lanes in (2, 4, 8) else "k_schedule_lds")
+    # the schedule kernel keeps round counts in registers for chains of <= 128 beads
+    sched_kernel = (f"k_schedule_reg<{ {2: 1, 4: 2, 8: 3}[lanes]}, {1 if nmax <= 64 else 2}>"
+                    if nmax <= 128 and lanes in (2, 4, 8) else "k_schedule_lds")
     sweep_keys = ([f"k_sweep_sched<{lanes}, {byte_t}>"] if lanes > 1 else [sweep_kernel]) + \
         ([sched_kernel] if pre else [])
     omega_key = (f"k_omega4<{spec.n_types}, {bits}>" if (spec.n_cells % 4 == 0)

@@ -651,24 +651,26 @@ size_t schedule_record_bytes(int nmax, int64_t chains) {
 }
 // Whether the register-count kernel takes chains of nmax beads in rounds of cap moves.
 bool schedule_in_registers(int nmax, int cap) {
-  return nmax <= 64 && (cap == 2 || cap == 4 || cap == 8);
+  return nmax <= 128 && (cap == 2 || cap == 4 || cap == 8);
 }
 size_t schedule_lds_bytes(int nmax, int cap) {
   return (size_t)(schedule_in_registers(nmax, cap) ? 2 * nmax + 2 : 4 * nmax + 5) * SCHED_LANES;
 }
 
-// Chains of at most 64 beads, rounds of at most cap = 2^CAPLOG moves.  The
-// number of moves in each round is kept in registers as CAPLOG + 1
-// bit-planes of 64 bits (bit l - 1 of plane j = bit j of round l's count),
-// so a move finds the first round with room by a bit scan of the top plane
-// (the full rounds) and counts itself by a ripple over the planes: the scan
-// waits for LDS once per move (its neighbours' rounds).  The rounds used
-// are always 1 .. max, at most one per move, so 64 bits hold them.  LDS
+// Chains of at most 64 NW beads (NW = 1 or 2), rounds of at most cap =
+// 2^CAPLOG moves.  The number of moves in each round is kept in registers as
+// CAPLOG + 1 bit-planes of 64 NW bits (bit l - 1 of plane j = bit j of round
+// l's count), so a move finds the first round with room by a bit scan of the
+// top plane (the full rounds) and counts itself by a ripple over the planes:
+// the scan waits for LDS once per move (its neighbours' rounds).  The rounds
+// used are always 1 .. max, at most one per move, so 64 NW bits hold them
+// (the two words of a plane are chosen by selects, never by a register
+// index, which would put the planes in scratch memory).  LDS
 // holds the round of each bead and of each move (2N + 2 bytes a chain); the
 // record goes to memory byte by byte, the bead of each move as the scan
 // picks it (staging the record in LDS for 16-byte stores costs waves, and
 // measured slower).
-template <int CAPLOG>
+template <int CAPLOG, int NW>
 __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __restrict__ Ps,
                                                               int nmax) {
   extern __shared__ uint8_t sh[];
@@ -695,7 +697,7 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
   const bool linear = kp.linear[p] != 0;
   int maxl = 0;
   int b = length > 0 ? pick(0) : 0;
-  uint64_t plane[CAPLOG + 1] = {};
+  uint64_t plane[CAPLOG + 1] = {}, plane_hi[CAPLOG + 1] = {};  // plane_hi: rounds 65 .. 128
   for (int i = 0; i < length; ++i) {
     const int cur = b;
     int l;
@@ -709,13 +711,29 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
       l += 1;
       if (i + 1 < length) b = pick(i + 1);
     }
-    l = __builtin_ctzll(~plane[CAPLOG] & (~0ull << (l - 1))) + 1;
-    uint64_t carry = 1ull << (l - 1);
+    uint64_t carry, carry_hi = 0;
+    if constexpr (NW == 1) {
+      l = __builtin_ctzll(~plane[CAPLOG] & (~0ull << (l - 1))) + 1;
+      carry = 1ull << (l - 1);
+    } else {
+      const int k = l - 1;
+      const uint64_t lo = k < 64 ? ~plane[CAPLOG] & (~0ull << k) : 0ull;
+      const uint64_t hi = ~plane_hi[CAPLOG] & (k < 64 ? ~0ull : ~0ull << (k - 64));
+      l = (lo ? __builtin_ctzll(lo) : 64 + __builtin_ctzll(hi)) + 1;
+      const uint64_t bit = 1ull << ((l - 1) & 63);
+      carry = l <= 64 ? bit : 0ull;
+      carry_hi = l <= 64 ? 0ull : bit;
+    }
 #pragma unroll
     for (int j = 0; j <= CAPLOG; ++j) {
       const uint64_t c = plane[j] & carry;
       plane[j] ^= carry;
       carry = c;
+      if constexpr (NW == 2) {
+        const uint64_t ch = plane_hi[j] & carry_hi;
+        plane_hi[j] ^= carry_hi;
+        carry_hi = ch;
+      }
     }
     at(W, cur + 1) = (uint8_t)l;
     at(lev, i) = (uint8_t)l;
@@ -731,7 +749,10 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
   for (int l = 1; l <= length; ++l) {
     int n = 0;
 #pragma unroll
-    for (int j = 0; j <= CAPLOG; ++j) n |= (int)((plane[j] >> (l - 1)) & 1u) << j;
+    for (int j = 0; j <= CAPLOG; ++j) {
+      const uint64_t w = NW == 1 || l <= 64 ? plane[j] : plane_hi[j];
+      n |= (int)((w >> ((l - 1) & 63)) & 1u) << j;
+    }
     if (l <= maxl) at(W, l) = (uint8_t)run;
     run += n;
     rec[sched_index(chain, stride, 2 + l)] = (uint8_t)(l <= maxl ? run : 0);
@@ -1601,8 +1622,10 @@ int launch_schedule(scmf_sys* s, int cap) {
   const size_t lds = schedule_lds_bytes(s->nmax, cap);
   const KParams* kp = s->d_kp;
   if (schedule_in_registers(s->nmax, cap)) {
-    const void* fn = cap == 2 ? (const void*)k_schedule_reg<1>
-                   : cap == 4 ? (const void*)k_schedule_reg<2> : (const void*)k_schedule_reg<3>;
+    const bool two = s->nmax > 64;
+    const void* fn = cap == 2 ? (two ? (const void*)k_schedule_reg<1, 2> : (const void*)k_schedule_reg<1, 1>)
+                   : cap == 4 ? (two ? (const void*)k_schedule_reg<2, 2> : (const void*)k_schedule_reg<2, 1>)
+                              : (two ? (const void*)k_schedule_reg<3, 2> : (const void*)k_schedule_reg<3, 1>);
     void* args[] = {(void*)&kp, (void*)&s->nmax};
     HIP_TRY(hipLaunchKernel(fn, grid, dim3(SCHED_LANES), args, lds, s->stream));
   } else {

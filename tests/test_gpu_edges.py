@@ -273,14 +273,15 @@ def test_gridding_follows_every_position_change(tmp_path, L, lanes):
         g.close()
 
 
-@pytest.mark.parametrize("lanes", [2, 8])
-def test_schedule_kernel_long_chains(lanes):
-    """70,000 diblocks of 96 beads: chains longer than 64 take the schedule
-    kernel that counts each round's moves in LDS (k_schedule_lds, records
-    staged in LDS), with rounds of 2 or 8 moves.  Sweeps bit-exact against
-    the oracle."""
-    banner(f"schedule kernel, N=96, lanes {lanes}")
-    spec = W.tiny(n_chains=70_000, N=96, grid=(32, 32, 32), L=(16.0, 16.0, 16.0))
+@pytest.mark.parametrize("N,lanes", [(96, 2), (96, 8), (128, 2), (160, 2), (160, 8)])
+def test_schedule_kernel_long_chains(N, lanes):
+    """70,000 diblocks of N beads, rounds of 2 or 8 moves.  Chains of 65-128
+    beads take the register-count schedule kernel with two 64-bit words per
+    bit-plane (rounds of 2 moves reach past round 64 there); longer chains take
+    the kernel that counts each round's moves in LDS (k_schedule_lds, records
+    staged in LDS).  Sweeps bit-exact against the oracle."""
+    banner(f"schedule kernel, N={N}, lanes {lanes}")
+    spec = W.tiny(n_chains=70_000, N=N, grid=(32, 32, 32), L=(16.0, 16.0, 16.0))
     run_pair(spec, sweeps=2, seed=31, density_bits=32, lanes_per_chain=lanes)
 
 
