@@ -119,6 +119,28 @@ __device__ __forceinline__ bool step_skipped(const KP& kp) {
   return kp.stop != NO_STOP && ((const uint32_t*)kp.ctr)[1] == 0u;
 }
 
+// Inside scmf_step two small launches ride on the step's other kernels: the
+// sweep clears the counts the step's gridding then fills (nothing reads them
+// in between), and the gridding advances the stream sweep word (nothing reads
+// it before the next sweep).  clear_counts runs in every workgroup of a sweep
+// launch, advance_sweep_word in one lane of a gridding launch.
+template <class KP>
+__device__ __forceinline__ void clear_counts(const KP& kp) {
+  SCMF_GLOBAL uint32_t* __restrict__ cw = gptr((uint32_t*)kp.counts);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < kp.count_words; q += stride)
+    cw[q] = 0u;
+}
+template <class KP>
+__device__ __forceinline__ void advance_sweep_word(const KP& kp) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    uint32_t* c = (uint32_t*)kp.ctr;
+    const bool ran = c[0] < kp.stop;
+    if (ran) c[0] = c[0] + 1u;
+    c[1] = ran ? 1u : 0u;
+  }
+}
+
 __device__ __forceinline__ BlockRef block_ref(const BlockRef* bt) {
   if (bt) return bt[blockIdx.x];
   BlockRef r;

@@ -363,10 +363,11 @@ __global__ void __launch_bounds__(64) k_advance(const KParams* __restrict__ Ps, 
 // mc_polymer_move_arr, one lane per chain, positions read and written in
 // global memory (used when a chain tile does not fit in LDS).
 __global__ void __launch_bounds__(BLOCK) k_sweep_global(const KParams* __restrict__ Ps,
-                                                        const BlockRef* __restrict__ bt) {
+                                                        const BlockRef* __restrict__ bt, int clear) {
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
   if (sweep_stopped(kp)) return;
+  if (clear) clear_counts(kp);
   SCMF_GLOBAL float* __restrict__ x = gptr((float*)kp.x);
   SCMF_GLOBAL float* __restrict__ y = gptr((float*)kp.y);
   SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
@@ -390,11 +391,13 @@ __global__ void __launch_bounds__(BLOCK) k_sweep_global(const KParams* __restric
 // per-lane bead index).  Each lane touches only its own column, so no barrier
 // is needed; global memory sees one read and one write of every coordinate.
 __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps,
-                                                  const BlockRef* __restrict__ bt, int nmax) {
+                                                  const BlockRef* __restrict__ bt, int nmax,
+                                                  int clear) {
   extern __shared__ float lds[];
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
   if (sweep_stopped(kp)) return;
+  if (clear) clear_counts(kp);
   SCMF_GLOBAL float* __restrict__ x = gptr((float*)kp.x);
   SCMF_GLOBAL float* __restrict__ y = gptr((float*)kp.y);
   SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
@@ -462,12 +465,14 @@ __device__ __forceinline__ int64_t sched_index(int64_t slot, int stride, int k) 
 
 template <int L, typename I>
 __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict__ Ps,
-                                                    const BlockRef* __restrict__ bt, int nmax) {
+                                                    const BlockRef* __restrict__ bt, int nmax,
+                                                    int clear) {
   constexpr int G = 64 / L;
   extern __shared__ float lds[];
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
   if (sweep_stopped(kp)) return;
+  if (clear) clear_counts(kp);
   SCMF_GLOBAL float* __restrict__ x = gptr((float*)kp.x);
   SCMF_GLOBAL float* __restrict__ y = gptr((float*)kp.y);
   SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
@@ -874,10 +879,11 @@ __global__ void __launch_bounds__(BLOCK) k_zero(uint32_t* __restrict__ p, int64_
 // gridding took 0.041 ms against 0.018 ms with one atomic per bead and
 // 0.017 ms with runs.)
 __global__ void __launch_bounds__(BLOCK) k_grid(const KParams* __restrict__ Ps,
-                                                const BlockRef* __restrict__ bt) {
+                                                const BlockRef* __restrict__ bt, int advance) {
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
   if (step_skipped(kp)) return;
+  if (advance) advance_sweep_word(kp);
   const SCMF_GLOBAL float* __restrict__ x = gptr((const float*)kp.x);
   const SCMF_GLOBAL float* __restrict__ y = gptr((const float*)kp.y);
   const SCMF_GLOBAL float* __restrict__ z = gptr((const float*)kp.z);
@@ -1637,12 +1643,15 @@ int launch_schedule(scmf_sys* s, int cap) {
   return SCMF_OK;
 }
 
-int launch_sweep(scmf_sys* s) {
+// clear: the sweep also clears the counts (scmf_step, whose gridding follows);
+// advance: k_advance moves the stream sweep word on (else the step's gridding does).
+int launch_sweep(scmf_sys* s, bool clear = false, bool advance = true) {
   const SweepVariant* v = s->variant;
   const BlockRef* none = nullptr;
+  int clr = clear ? 1 : 0;
   if (v->L == 0) {
     hipLaunchKernelGGL(k_sweep_global, dim3(grid_for(s->n_chains)), dim3(BLOCK), 0, s->stream,
-                       (const KParams*)s->d_kp, none);
+                       (const KParams*)s->d_kp, none, clr);
   } else {
     const int G = 64 / v->L;
     const dim3 grid((unsigned)((s->n_chains + G - 1) / G));
@@ -1651,10 +1660,11 @@ int launch_sweep(scmf_sys* s) {
       if (rc != SCMF_OK) return rc;
     }
     const KParams* kpp = s->d_kp;
-    void* args[] = {(void*)&kpp, (void*)&none, (void*)&s->nmax};
+    void* args[] = {(void*)&kpp, (void*)&none, (void*)&s->nmax, (void*)&clr};
     HIP_TRY(hipLaunchKernel(v->fn, grid, dim3(64), args, s->lds_bytes, s->stream));
   }
-  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s->stream, (const KParams*)s->d_kp, 1);
+  if (advance)
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s->stream, (const KParams*)s->d_kp, 1);
   HIP_TRY(hipGetLastError());
   return SCMF_OK;
 }
@@ -1883,7 +1893,8 @@ int sort_chains(scmf_sys* s) {
 // counts must already be cleared).
 // use_cells: read the bead cells the last sweep stored (valid only while
 // s->cells_fresh holds) instead of binning the positions.
-int launch_tiled_grid(scmf_sys* s, hipStream_t st, const KParams* Ps, int sys, bool use_cells) {
+int launch_tiled_grid(scmf_sys* s, hipStream_t st, const KParams* Ps, int sys, bool use_cells,
+                      int advance) {
   const dim3 tiles((unsigned)s->geo.ntiles);
   const void* bin = grid_tiles_kernel(use_cells, s->opts.density_bits == 32, s->kp.pow2 != 0,
                                       s->T == 2);
@@ -1894,14 +1905,67 @@ int launch_tiled_grid(scmf_sys* s, hipStream_t st, const KParams* Ps, int sys, b
   if (s->T == 2) {
     auto sum = s->opts.density_bits == 32 ? k_tiles_sum_pair<32> : k_tiles_sum_pair<16>;
     hipLaunchKernelGGL(sum, tiles, dim3(SUM_THREADS), 0, st, Ps, sys, s->geo,
-                       (const uint32_t*)s->images);
+                       (const uint32_t*)s->images, advance);
     HIP_TRY(hipGetLastError());
     return SCMF_OK;
   }
   auto sum = s->opts.density_bits == 32 ? k_tiles_sum<32> : k_tiles_sum<16>;
   hipLaunchKernelGGL(sum, tiles, dim3(SUM_THREADS), 0, st, Ps, sys, s->geo,
-                     (const uint16_t*)s->images);
+                     (const uint16_t*)s->images, advance);
   HIP_TRY(hipGetLastError());
+  return SCMF_OK;
+}
+
+// n sweeps; fused: the first clears the counts and leaves the stream word to
+// the step's gridding (scmf_step, n = 1).
+int sweeps_impl(scmf_sys* s, int32_t n_sweeps, bool fused) {
+  for (int32_t i = 0; i < n_sweeps; ++i) {
+    Timed t;
+    if (s->timing) {
+      int rc = s->ev_begin(K_SWEEP, &t);
+      if (rc) return rc;
+    }
+    int rc = launch_sweep(s, fused, !fused);
+    if (rc) return rc;
+    HIP_TRY(hipGetLastError());
+    s->cells_fresh = s->kp.cells != nullptr && s->variant->L >= 2;
+    if (s->timing) {
+      int rc = s->ev_end(&t);
+      if (rc) return rc;
+    }
+    s->sweeps += 1;
+    s->moves += (uint64_t)s->n_beads;
+  }
+  return SCMF_OK;
+}
+
+// The gridding; clear: clear the counts first (else the step's sweep did);
+// advance: move the stream sweep word on (the step's sweep left it).
+int grid_impl(scmf_sys* s, bool clear, bool advance) {
+  const int adv = advance ? 1 : 0;
+  if (s->tile_lds && !s->sorted && s->sort_every > 0) {
+    int rc = sort_chains(s);
+    if (rc) return rc;
+  }
+  Timed t;
+  if (s->timing) {
+    int rc = s->ev_begin(K_GRID, &t);
+    if (rc) return rc;
+  }
+  if (clear)
+    hipLaunchKernelGGL(k_zero_counts, dim3(grid_strided(s->count_words)), dim3(BLOCK), 0, s->stream,
+                       (const KParams*)s->d_kp, (const BlockRef*)nullptr);
+  const bool b32 = s->opts.density_bits == 32;
+  if (s->tile_lds && s->sorted) {
+    if (s->cells_fresh) s->n_cell_grids += 1;
+    int rc = launch_tiled_grid(s, s->stream, s->d_kp, 0, s->cells_fresh, adv);
+    if (rc) return rc;
+  } else {
+    hipLaunchKernelGGL(k_grid, dim3(grid_strided(s->n_beads)), dim3(BLOCK), 0, s->stream,
+                       (const KParams*)s->d_kp, (const BlockRef*)nullptr, adv);
+  }
+  HIP_TRY(hipGetLastError());
+  if (s->timing) return s->ev_end(&t);
   return SCMF_OK;
 }
 
@@ -2306,51 +2370,12 @@ void scmf_destroy(scmf_sys* s) {
 int scmf_mc_sweep(scmf_sys* s, int32_t n_sweeps) {
   if (!s) return fail(SCMF_E_ARG, "null handle");
   if (n_sweeps < 0) return fail(SCMF_E_ARG, "n_sweeps must be non-negative");
-  for (int32_t i = 0; i < n_sweeps; ++i) {
-    Timed t;
-    if (s->timing) {
-      int rc = s->ev_begin(K_SWEEP, &t);
-      if (rc) return rc;
-    }
-    int rc = launch_sweep(s);
-    if (rc) return rc;
-    HIP_TRY(hipGetLastError());
-    s->cells_fresh = s->kp.cells != nullptr && s->variant->L >= 2;
-    if (s->timing) {
-      int rc = s->ev_end(&t);
-      if (rc) return rc;
-    }
-    s->sweeps += 1;
-    s->moves += (uint64_t)s->n_beads;
-  }
-  return SCMF_OK;
+  return sweeps_impl(s, n_sweeps, false);
 }
 
 int scmf_grid_density(scmf_sys* s) {
   if (!s) return fail(SCMF_E_ARG, "null handle");
-  if (s->tile_lds && !s->sorted && s->sort_every > 0) {
-    int rc = sort_chains(s);
-    if (rc) return rc;
-  }
-  Timed t;
-  if (s->timing) {
-    int rc = s->ev_begin(K_GRID, &t);
-    if (rc) return rc;
-  }
-  hipLaunchKernelGGL(k_zero_counts, dim3(grid_strided(s->count_words)), dim3(BLOCK), 0, s->stream,
-                     (const KParams*)s->d_kp, (const BlockRef*)nullptr);
-  const bool b32 = s->opts.density_bits == 32;
-  if (s->tile_lds && s->sorted) {
-    if (s->cells_fresh) s->n_cell_grids += 1;
-    int rc = launch_tiled_grid(s, s->stream, s->d_kp, 0, s->cells_fresh);
-    if (rc) return rc;
-  } else {
-    hipLaunchKernelGGL(k_grid, dim3(grid_strided(s->n_beads)), dim3(BLOCK), 0, s->stream,
-                       (const KParams*)s->d_kp, (const BlockRef*)nullptr);
-  }
-  HIP_TRY(hipGetLastError());
-  if (s->timing) return s->ev_end(&t);
-  return SCMF_OK;
+  return grid_impl(s, true, false);
 }
 
 int scmf_sort_chains(scmf_sys* s) {
@@ -2387,9 +2412,11 @@ int scmf_step(scmf_sys* s, int32_t step) {
     int rc = sort_chains(s);
     if (rc) return rc;
   }
-  int rc = scmf_mc_sweep(s, 1);
+  // the sweep clears the counts and the gridding advances the stream word
+  // (two launches fewer than scmf_mc_sweep + scmf_grid_density)
+  int rc = sweeps_impl(s, 1, true);
   if (rc) return rc;
-  if ((rc = scmf_grid_density(s))) return rc;
+  if ((rc = grid_impl(s, false, true))) return rc;
   if (s->opts.omega_every_step || (step % 10)) return scmf_update_omega(s);
   return SCMF_OK;
 }
@@ -3404,11 +3431,11 @@ int batch_build(scmf_batch* b, bool room) {
   HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
   for (const scmf_batch::Group& g : b->groups) {
     if (g.v->L == 0) {
-      hipLaunchKernelGGL(k_sweep_global, dim3(g.nblocks), dim3(BLOCK), 0, b->cap, Ps, table(g.bt));
+      hipLaunchKernelGGL(k_sweep_global, dim3(g.nblocks), dim3(BLOCK), 0, b->cap, Ps, table(g.bt), 0);
     } else {
       const BlockRef* bt = table(g.bt);
-      int nmax = g.nmax;
-      void* args[] = {(void*)&Ps, (void*)&bt, (void*)&nmax};
+      int nmax = g.nmax, clear = 0;
+      void* args[] = {(void*)&Ps, (void*)&bt, (void*)&nmax, (void*)&clear};
       (void)hipLaunchKernel(g.v->fn, dim3(g.nblocks), dim3(64), args, g.lds, b->cap);
     }
   }
@@ -3416,10 +3443,10 @@ int batch_build(scmf_batch* b, bool room) {
   hipLaunchKernelGGL(k_advance, dim3((unsigned)((nk + 63) / 64)), dim3(64), 0, b->cap, Ps, nk);
   hipLaunchKernelGGL(k_zero_counts, dim3(b->n_zero), dim3(BLOCK), 0, b->cap, Ps, table(b->bt_zero));
   if (b->n_grid)
-    hipLaunchKernelGGL(k_grid, dim3(b->n_grid), dim3(BLOCK), 0, b->cap, Ps, table(b->bt_grid));
+    hipLaunchKernelGGL(k_grid, dim3(b->n_grid), dim3(BLOCK), 0, b->cap, Ps, table(b->bt_grid), 0);
   int trc = SCMF_OK;
   for (int i = 0; i < n && trc == SCMF_OK; ++i)
-    if (b->m[i]->tile_lds && b->m[i]->sorted) trc = launch_tiled_grid(b->m[i], b->cap, Ps, i, false);
+    if (b->m[i]->tile_lds && b->m[i]->sorted) trc = launch_tiled_grid(b->m[i], b->cap, Ps, i, false, 0);
   hipLaunchKernelGGL(k_omega, dim3(b->n_omega), dim3(BLOCK), 0, b->cap, Ps, table(b->bt_omega), 1);
   hipError_t e = hipStreamEndCapture(b->cap, &b->graph);
   if (trc) return trc;

@@ -634,9 +634,11 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
 template <int BITS>
 __global__ void __launch_bounds__(SUM_THREADS) k_tiles_sum(const KParams* __restrict__ Ps, int sys,
                                                    TileGeo g,
-                                                   const uint16_t* __restrict__ images) {
+                                                   const uint16_t* __restrict__ images,
+                                                   int advance) {
   KPC& kp = param(Ps, (uint32_t)sys);
   if (step_skipped(kp)) return;
+  if (advance) advance_sweep_word(kp);
   SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
   SCMF_GLOBAL uint16_t* __restrict__ counts16 = gptr((uint16_t*)kp.counts);
   uint32_t* __restrict__ overflow = (uint32_t*)kp.overflow;
@@ -711,9 +713,11 @@ __global__ void __launch_bounds__(SUM_THREADS) k_tiles_sum(const KParams* __rest
 template <int BITS>
 __global__ void __launch_bounds__(SUM_THREADS) k_tiles_sum_pair(const KParams* __restrict__ Ps,
                                                                 int sys, TileGeo g,
-                                                                const uint32_t* __restrict__ img) {
+                                                                const uint32_t* __restrict__ img,
+                                                                int advance) {
   KPC& kp = param(Ps, (uint32_t)sys);
   if (step_skipped(kp)) return;
+  if (advance) advance_sweep_word(kp);
   SCMF_GLOBAL uint32_t* __restrict__ counts = gptr((uint32_t*)kp.counts);
   SCMF_GLOBAL uint16_t* __restrict__ counts16 = gptr((uint16_t*)kp.counts);
   uint32_t* __restrict__ overflow = (uint32_t*)kp.overflow;
