@@ -223,7 +223,7 @@ def batched_vs_plain(steps, warmup, n_systems=64):
     batch_members = members()
     b = scmf.Batch(batch_members)
     best = None
-    for lanes in (1, 2, 4, 8):           # autotune the shared chain-to-wavefront shape
+    for lanes in (1, 2, 4, 8, 16):       # autotune the shared chain-to-wavefront shape
         b.set_lanes(lanes)
         b.step(1)
         b.sync()
@@ -547,6 +547,7 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
     measured = profile_json(workload, "traffic").get("bytes_per_launch", {})
     byte_t = "unsigned char" if nmax <= 255 else "unsigned short"
     # the schedule kernel keeps round counts in registers for chains of <= 128 beads
+    # and rounds of 2, 4 or 8 moves
     sched_kernel = (f"k_schedule_reg<{ {2: 1, 4: 2, 8: 3}[lanes]}, {1 if nmax <= 64 else 2}>"
                     if nmax <= 128 and lanes in (2, 4, 8) else "k_schedule_lds")
     sweep_keys = ([f"k_sweep_sched<{lanes}, {byte_t}>"] if lanes > 1 else [sweep_kernel]) + \

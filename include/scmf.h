@@ -72,7 +72,7 @@ typedef struct scmf_opts {
   int32_t init_coords;      /* 1: build the initial Gaussian walks on the device
                                (initialize_coord, device_function.cu:140-207);
                                0: the caller supplies positions (scmf_copy_in_pos) */
-  int32_t lanes_per_chain;  /* sweep kernel shape: 1, 2, 4 or 8 lanes share a chain
+  int32_t lanes_per_chain;  /* sweep kernel shape: 1, 2, 4, 8 or 16 lanes share a chain
                                staged in LDS; -1 = one lane per chain in global
                                memory; 0 = pick automatically.  Results do not
                                depend on it. */

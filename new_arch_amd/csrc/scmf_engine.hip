@@ -1570,6 +1570,11 @@ const SweepVariant kVariants[] = {
     {2, true, (const void*)k_sweep_sched<2, uint16_t>, sched_lds_bytes<2, uint16_t>},
     {4, true, (const void*)k_sweep_sched<4, uint16_t>, sched_lds_bytes<4, uint16_t>},
     {8, true, (const void*)k_sweep_sched<8, uint16_t>, sched_lds_bytes<8, uint16_t>},
+    // 16 lanes a chain, 4 chains a wave: twice the waves of 8 lanes for small
+    // systems of short chains, whose rounds hardly grow (N = 32: 6.0 against
+    // 6.1 rounds a sweep)
+    {16, false, (const void*)k_sweep_sched<16, uint8_t>, sched_lds_bytes<16, uint8_t>},
+    {16, true, (const void*)k_sweep_sched<16, uint16_t>, sched_lds_bytes<16, uint16_t>},
 };
 
 // Pick the variant: the requested lanes_per_chain if it fits, else the
@@ -1607,7 +1612,7 @@ int choose_sweep(scmf_sys* s) {
     }
   }
   if (s->opts.lanes_per_chain != 0 && best->L != s->opts.lanes_per_chain)
-    return fail(SCMF_E_ARG, "lanes_per_chain must be 1, 2, 4 or 8 and fit in LDS");
+    return fail(SCMF_E_ARG, "lanes_per_chain must be 1, 2, 4, 8 or 16 and fit in LDS");
   const size_t need = best->bytes(s->nmax);
   if (need > 65536) {
     hipError_t e = raise_lds_limit(best->fn, need);
@@ -2772,7 +2777,7 @@ int scmf_autotune(scmf_sys* s, int32_t sweeps_per_variant, int32_t* chosen, floa
   HIP_TRY(hipEventCreate(&a));
   HIP_TRY(hipEventCreate(&b));
   int rc = SCMF_OK;
-  for (int32_t L : {-1, 1, 2, 4, 8}) {
+  for (int32_t L : {-1, 1, 2, 4, 8, 16}) {
     s->opts.lanes_per_chain = L;
     if (choose_sweep(s) != SCMF_OK) continue;
     if ((rc = launch_sweep(s))) break;  // warm-up launch

@@ -42,7 +42,7 @@ def same(a, b):
                           np.ascontiguousarray(b).view(np.uint32))
 
 
-@pytest.mark.parametrize("lanes", [0, 1, 4])
+@pytest.mark.parametrize("lanes", [0, 1, 4, 16])
 def test_batch_equals_members_alone(lanes):
     banner(f"batch vs alone lanes={lanes}")
     systems = mix()

@@ -273,7 +273,7 @@ def test_gridding_follows_every_position_change(tmp_path, L, lanes):
         g.close()
 
 
-@pytest.mark.parametrize("N,lanes", [(96, 2), (96, 8), (128, 2), (160, 2), (160, 8)])
+@pytest.mark.parametrize("N,lanes", [(96, 2), (96, 8), (96, 16), (128, 2), (160, 2), (160, 8)])
 def test_schedule_kernel_long_chains(N, lanes):
     """70,000 diblocks of N beads, rounds of 2 or 8 moves.  Chains of 65-128
     beads take the register-count schedule kernel with two 64-bit words per

@@ -113,7 +113,7 @@ def test_omega_reference_dumps_on_gpu(golden, step):
     g.close()
 
 
-@pytest.mark.parametrize("lanes", [0, -1, 1, 2, 4, 8])
+@pytest.mark.parametrize("lanes", [0, -1, 1, 2, 4, 8, 16])
 @pytest.mark.parametrize("move", ["smart", "trial"])
 @pytest.mark.parametrize("name", list(SPECS))
 def test_sweep_bit_exact(name, move, lanes):
@@ -183,7 +183,7 @@ def test_autotune_leaves_trajectory_unchanged():
     g.grid_density()
     g.update_omega()
     lanes, ms = g.autotune(2)
-    assert lanes in (-1, 1, 2, 4, 8) and ms > 0
+    assert lanes in (-1, 1, 2, 4, 8, 16) and ms > 0
     assert g.stats().accepts == 0 and g.stats().sweeps == 0
     pos = o.init_coords()
     om = g.omega()
