@@ -555,10 +555,10 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
     omega_key = (f"k_omega4<{spec.n_types}, {bits}>" if (spec.n_cells % 4 == 0)
                  else "k_omega")
     pair = "true" if spec.n_types == 2 else "false"
+    # the step's gridding (its counts cleared by the step's sweep)
     grid_keys = [f"k_grid_tiles<{bits}, false, true, {pair}>" if from_cells
                  else f"k_grid_tiles<{bits}, {'true' if spec_pow2(spec) else 'false'}, false, {pair}>",
-                 f"k_tiles_sum_pair<{bits}>" if spec.n_types == 2 else f"k_tiles_sum<{bits}>",
-                 "k_zero_counts"]
+                 f"k_tiles_sum_pair<{bits}>" if spec.n_types == 2 else f"k_tiles_sum<{bits}>"]
 
     def traffic_of(keys):
         if all(k in measured for k in keys):
@@ -576,9 +576,9 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
 
     passes = {"grid": dict(roof(grid_bytes_cells, tm.grid_ms,
                                 traffic_of(grid_keys) if tiled else None),
-                           kernel=("k_zero_counts+k_grid_tiles" + ("(cells)+" if from_cells else "+")
+                           kernel=("k_grid_tiles" + ("(cells)+" if from_cells else "+")
                                    + ("k_tiles_sum_pair" if spec.n_types == 2 else "k_tiles_sum"))
-                           if tiled else "k_zero_counts+k_grid",
+                           if tiled else "k_grid",
                            bytes_note="cells read (4 B a bead) + counts written" if from_cells
                            else "positions read (12 B a bead) + counts written"),
               "omega": dict(roof(omega_bytes, tm.omega_ms, traffic_of([omega_key])),
