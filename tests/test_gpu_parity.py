@@ -203,6 +203,35 @@ def tiled_spec(grid, T=2, N=16):
                       nref=N, polymers=poly)
 
 
+@pytest.mark.parametrize("cadence", [0, 1])
+@pytest.mark.parametrize("bits", [16, 32])
+def test_step_loop_on_ordered_two_type_system(bits, cadence):
+    """scmf_step on a spatially ordered two-type system (the sweep clearing the
+    counts, the tiled gridding advancing the stream word, orderings every 3
+    sweeps): positions, counts and omega equal the oracle's after every step
+    of both cadences (omega kept on steps 10 and 20 with the reference
+    cadence)."""
+    banner(f"ordered step loop u{bits} cadence={cadence}")
+    spec = tiled_spec((32, 32, 32), T=2)
+    g, o = pair(spec, seed=41, density_bits=bits, sort_every=3, omega_every_step=bool(cadence))
+    pos = o.init_coords()
+    g.grid_density()
+    g.update_omega()
+    om = o.omega(o.grid(pos)).astype(np.float32)
+    assert same_bits(g.omega(), om)
+    for step in range(1, 26):
+        g.step(step)
+        o.sweep(pos, om, step - 1)
+        counts = o.grid(pos)
+        if cadence or step % 10:
+            om = o.omega(counts).astype(np.float32)
+        assert same_bits(g.positions(), pos), f"step {step} positions"
+        np.testing.assert_array_equal(g.density(), counts, err_msg=f"step {step} counts")
+        assert same_bits(g.omega(), om), f"step {step} omega"
+    assert g.stats().sorts >= 8
+    g.close()
+
+
 @pytest.mark.parametrize("bits", [16, 32])
 @pytest.mark.parametrize("grid,T", [((27, 31, 40), 2), ((32, 32, 32), 3), ((64, 24, 33), 2),
                                     ((48, 40, 10), 2), ((40, 36, 30), 1)])
