@@ -363,7 +363,8 @@ __global__ void __launch_bounds__(64) k_advance(const KParams* __restrict__ Ps, 
 // mc_polymer_move_arr, one lane per chain, positions read and written in
 // global memory (used when a chain tile does not fit in LDS).
 __global__ void __launch_bounds__(BLOCK) k_sweep_global(const KParams* __restrict__ Ps,
-                                                        const BlockRef* __restrict__ bt, int clear) {
+                                                        const BlockRef* __restrict__ bt, int clear,
+                                                        int64_t c0, int64_t c1) {
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
   if (sweep_stopped(kp)) return;
@@ -373,9 +374,9 @@ __global__ void __launch_bounds__(BLOCK) k_sweep_global(const KParams* __restric
   SCMF_GLOBAL float* __restrict__ z = gptr((float*)kp.z);
   const SCMF_GLOBAL float* __restrict__ omega = gptr((const float*)kp.omega);
   const uint32_t sweep = ((const uint32_t*)kp.ctr)[0];
-  const int64_t chain = (int64_t)br.local * BLOCK + threadIdx.x;
+  const int64_t chain = c0 + (int64_t)br.local * BLOCK + threadIdx.x;
   uint32_t acc = 0;
-  if (chain < kp.n_chains) {
+  if (chain < c1 && chain < kp.n_chains) {
     const int p = arch_of_chain(kp, chain);
     const int length = kp.len[p];
     const int64_t off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
@@ -392,7 +393,7 @@ __global__ void __launch_bounds__(BLOCK) k_sweep_global(const KParams* __restric
 // is needed; global memory sees one read and one write of every coordinate.
 __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps,
                                                   const BlockRef* __restrict__ bt, int nmax,
-                                                  int clear) {
+                                                  int clear, int64_t c0, int64_t c1) {
   extern __shared__ float lds[];
   const BlockRef br = block_ref(bt);
   KPC& kp = param(Ps, br.sys);
@@ -404,9 +405,9 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
   const SCMF_GLOBAL float* __restrict__ omega = gptr((const float*)kp.omega);
   const uint32_t sweep = ((const uint32_t*)kp.ctr)[0];
   const int lane = threadIdx.x;
-  const int64_t chain = (int64_t)br.local * 64 + lane;
+  const int64_t chain = c0 + (int64_t)br.local * 64 + lane;
   uint32_t acc = 0;
-  if (chain < kp.n_chains) {
+  if (chain < c1 && chain < kp.n_chains) {
     const int p = arch_of_chain(kp, chain);
     const int length = kp.len[p];
     const int64_t off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
@@ -466,7 +467,7 @@ __device__ __forceinline__ int64_t sched_index(int64_t slot, int stride, int k) 
 template <int L, typename I>
 __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict__ Ps,
                                                     const BlockRef* __restrict__ bt, int nmax,
-                                                    int clear) {
+                                                    int clear, int64_t c0, int64_t c1) {
   constexpr int G = 64 / L;
   extern __shared__ float lds[];
   const BlockRef br = block_ref(bt);
@@ -495,8 +496,8 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
   I* list = start + nmax + 2;
   I* rnd = list + nmax;   // the round of each move (in-sweep scan only)
   I* bead = list + nmax;  // the bead of each move (schedules from the schedule kernels)
-  const int64_t chain = (int64_t)br.local * G + slot;
-  const bool active = chain < kp.n_chains;
+  const int64_t chain = c0 + (int64_t)br.local * G + slot;
+  const bool active = chain < c1 && chain < kp.n_chains;
   int p = 0, length = 0;
   int64_t off = 0;
   uint32_t id = 0;
@@ -677,12 +678,12 @@ size_t schedule_lds_bytes(int nmax, int cap) {
 // measured slower).
 template <int CAPLOG, int NW>
 __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __restrict__ Ps,
-                                                              int nmax) {
+                                                              int nmax, int64_t c0, int64_t c1) {
   extern __shared__ uint8_t sh[];
   KPC& kp = param(Ps, 0);
   const int t = threadIdx.x;
-  const int64_t chain = (int64_t)blockIdx.x * SCHED_LANES + t;
-  if (chain >= kp.n_chains) return;
+  const int64_t chain = c0 + (int64_t)blockIdx.x * SCHED_LANES + t;
+  if (chain >= c1) return;
   const uint32_t sweep = ((const uint32_t*)kp.ctr)[0];
   const int p = arch_of_chain(kp, chain);
   const int length = kp.len[p];
@@ -1347,6 +1348,11 @@ struct scmf_sys {
   int sort_every = 0;
   uint64_t n_sorts = 0;
   uint64_t n_cell_grids = 0;
+  struct Run {
+    int64_t c0, c1;             // chains [c0, c1) of consecutive architectures
+    int nmax;                   // their longest chain
+  };
+  std::vector<Run> runs;        // the sweep's launches
   KParams* d_kp = nullptr;  // device copy of kp read by the batch-capable kernels
   uint32_t* d_ctr = nullptr;
   size_t lds_bytes = 0; // LDS of the sweep kernel
@@ -1628,18 +1634,25 @@ int choose_sweep(scmf_sys* s) {
 }
 
 // k_schedule_reg when its register counts hold the rounds, else k_schedule_lds.
-int launch_schedule(scmf_sys* s, int cap) {
-  const dim3 grid((unsigned)((s->n_chains + SCHED_LANES - 1) / SCHED_LANES));
-  const size_t lds = schedule_lds_bytes(s->nmax, cap);
+// The schedules of the chains in [c0, c1), whose longest has nmax beads, by
+// the register-count kernel, which stores only its own chains' record bytes;
+// when that kernel cannot take them, the LDS kernel schedules every chain of
+// the system (its 16-byte stores write the records of whole 64-chain blocks),
+// so the caller then launches it once.
+int launch_schedule(scmf_sys* s, int cap, int64_t c0, int64_t c1, int nmax) {
   const KParams* kp = s->d_kp;
-  if (schedule_in_registers(s->nmax, cap)) {
-    const bool two = s->nmax > 64;
+  if (schedule_in_registers(nmax, cap)) {
+    const dim3 grid((unsigned)((c1 - c0 + SCHED_LANES - 1) / SCHED_LANES));
+    const size_t lds = schedule_lds_bytes(nmax, cap);
+    const bool two = nmax > 64;
     const void* fn = cap == 2 ? (two ? (const void*)k_schedule_reg<1, 2> : (const void*)k_schedule_reg<1, 1>)
                    : cap == 4 ? (two ? (const void*)k_schedule_reg<2, 2> : (const void*)k_schedule_reg<2, 1>)
                               : (two ? (const void*)k_schedule_reg<3, 2> : (const void*)k_schedule_reg<3, 1>);
-    void* args[] = {(void*)&kp, (void*)&s->nmax};
+    void* args[] = {(void*)&kp, (void*)&nmax, (void*)&c0, (void*)&c1};
     HIP_TRY(hipLaunchKernel(fn, grid, dim3(SCHED_LANES), args, lds, s->stream));
   } else {
+    const dim3 grid((unsigned)((s->n_chains + SCHED_LANES - 1) / SCHED_LANES));
+    const size_t lds = schedule_lds_bytes(s->nmax, cap);
     if (lds > 65536) HIP_TRY(raise_lds_limit((const void*)k_schedule_lds, lds));
     void* args[] = {(void*)&kp, (void*)&s->nmax, (void*)&cap};
     HIP_TRY(hipLaunchKernel((const void*)k_schedule_lds, grid, dim3(SCHED_LANES), args, lds,
@@ -1650,23 +1663,43 @@ int launch_schedule(scmf_sys* s, int cap) {
 
 // clear: the sweep also clears the counts (scmf_step, whose gridding follows);
 // advance: k_advance moves the stream sweep word on (else the step's gridding does).
+// The LDS shapes sweep each run of architectures (s->runs) with its own
+// launch, the chain tiles sized to the run's longest chain, so a few long
+// chains do not cut the waves of all the others.
 int launch_sweep(scmf_sys* s, bool clear = false, bool advance = true) {
   const SweepVariant* v = s->variant;
   const BlockRef* none = nullptr;
-  int clr = clear ? 1 : 0;
+  const KParams* kpp = s->d_kp;
   if (v->L == 0) {
+    int clr = clear ? 1 : 0;
+    int64_t c0 = 0, c1 = s->n_chains;
     hipLaunchKernelGGL(k_sweep_global, dim3(grid_for(s->n_chains)), dim3(BLOCK), 0, s->stream,
-                       (const KParams*)s->d_kp, none, clr);
+                       kpp, none, clr, c0, c1);
   } else {
     const int G = 64 / v->L;
-    const dim3 grid((unsigned)((s->n_chains + G - 1) / G));
     if (s->pre_sched) {
-      const int rc = launch_schedule(s, v->L);
-      if (rc != SCMF_OK) return rc;
+      bool split = true;
+      for (const auto& r : s->runs) split = split && schedule_in_registers(r.nmax, v->L);
+      if (split) {
+        for (const auto& r : s->runs) {
+          const int rc = launch_schedule(s, v->L, r.c0, r.c1, r.nmax);
+          if (rc != SCMF_OK) return rc;
+        }
+      } else {
+        const int rc = launch_schedule(s, v->L, 0, s->n_chains, s->nmax);
+        if (rc != SCMF_OK) return rc;
+      }
     }
-    const KParams* kpp = s->d_kp;
-    void* args[] = {(void*)&kpp, (void*)&none, (void*)&s->nmax, (void*)&clr};
-    HIP_TRY(hipLaunchKernel(v->fn, grid, dim3(64), args, s->lds_bytes, s->stream));
+    for (size_t k = 0; k < s->runs.size(); ++k) {
+      int nm = s->runs[k].nmax;
+      int clr = clear && k == 0 ? 1 : 0;  // the first launch clears every count
+      int64_t c0 = s->runs[k].c0, c1 = s->runs[k].c1;
+      const size_t lds = s->pre_sched && v->L >= 2 ? sched_pre_lds_bytes(v->L, nm) : v->bytes(nm);
+      if (lds > 65536) HIP_TRY(raise_lds_limit(v->fn, lds));
+      const dim3 grid((unsigned)((c1 - c0 + G - 1) / G));
+      void* args[] = {(void*)&kpp, (void*)&none, (void*)&nm, (void*)&clr, (void*)&c0, (void*)&c1};
+      HIP_TRY(hipLaunchKernel(v->fn, grid, dim3(64), args, lds, s->stream));
+    }
   }
   if (advance)
     hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s->stream, (const KParams*)s->d_kp, 1);
@@ -2039,6 +2072,20 @@ int create_impl(scmf_sys** out, const HostDesc& hd, const scmf_opts* opts) {
   // Schedules in their own kernel where the in-sweep scan outweighs a launch
   // (measured: slower for systems of a few thousand chains, batches included).
   s->pre_sched = s->nmax <= 255 && s->n_chains >= 65536;
+  // sweep runs: consecutive architectures of one chain length share a launch
+  // sized to that length (large systems; small ones keep one launch)
+  s->runs.clear();
+  for (int p = 0; p < s->kp.P; ++p) {
+    const int64_t c0 = s->kp.chain_base[p], c1 = s->kp.chain_base[p + 1];
+    if (c1 <= c0) continue;
+    const int len = s->kp.len[p];
+    if (!s->runs.empty() && (s->runs.back().nmax == len || s->n_chains < 65536)) {
+      s->runs.back().c1 = c1;
+      s->runs.back().nmax = std::max(s->runs.back().nmax, len);
+    } else {
+      s->runs.push_back({c0, c1, len});
+    }
+  }
   if ((rc = choose_sweep(s))) return bail(rc);
   const int64_t B = s->n_beads;
   const int64_t TC = (int64_t)s->T * s->n_cells;
@@ -3436,11 +3483,13 @@ int batch_build(scmf_batch* b, bool room) {
   HIP_TRY(hipStreamBeginCapture(b->cap, hipStreamCaptureModeRelaxed));
   for (const scmf_batch::Group& g : b->groups) {
     if (g.v->L == 0) {
-      hipLaunchKernelGGL(k_sweep_global, dim3(g.nblocks), dim3(BLOCK), 0, b->cap, Ps, table(g.bt), 0);
+      hipLaunchKernelGGL(k_sweep_global, dim3(g.nblocks), dim3(BLOCK), 0, b->cap, Ps, table(g.bt), 0,
+                         (int64_t)0, INT64_MAX);
     } else {
       const BlockRef* bt = table(g.bt);
       int nmax = g.nmax, clear = 0;
-      void* args[] = {(void*)&Ps, (void*)&bt, (void*)&nmax, (void*)&clear};
+      int64_t c0 = 0, c1 = INT64_MAX;  // every chain of each member
+      void* args[] = {(void*)&Ps, (void*)&bt, (void*)&nmax, (void*)&clear, (void*)&c0, (void*)&c1};
       (void)hipLaunchKernel(g.v->fn, dim3(g.nblocks), dim3(64), args, g.lds, b->cap);
     }
   }
