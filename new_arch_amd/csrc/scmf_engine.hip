@@ -507,11 +507,28 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
     off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
     id = gptr((const uint32_t*)kp.slot_chain)[chain];
   }
-  // load the tile
-  for (int b = r; b < length; b += L) {
-    sx[b * G + slot] = x[off + b];
-    sy[b * G + slot] = y[off + b];
-    sz[b * G + slot] = z[off + b];
+  // load the tile, eight beads a lane in flight at once (the loop with one
+  // bead an iteration waited for each: 0.18 ms more a sweep at 268M beads)
+  for (int b0 = r; b0 < length; b0 += 8 * L) {
+    float tx[8], ty[8], tz[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int b = b0 + u * L;
+      if (b < length) {
+        tx[u] = x[off + b];
+        ty[u] = y[off + b];
+        tz[u] = z[off + b];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int b = b0 + u * L;
+      if (b < length) {
+        sx[b * G + slot] = tx[u];
+        sy[b * G + slot] = ty[u];
+        sz[b * G + slot] = tz[u];
+      }
+    }
   }
   // the wave's eight chains are one record group (records interleave groups
   // of 8 slots), all nmax beads long: their entries load 8 bytes at a time
