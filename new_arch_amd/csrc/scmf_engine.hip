@@ -678,7 +678,8 @@ size_t sched_pre_lds_bytes(int L, int nmax) {
 // chain).  The picks depend on the streams only, so the schedule does not
 // wait for positions.  Record of a slot (uint8, chains of at most 255
 // beads): entry 0 the number of rounds, entries 1 .. N+2 start[0 .. N+1]
-// (the first list entry of each round; zero after the last round), entries
+// (the first list entry of each round, through start[maxl + 1]; the entries
+// after it are not read), entries
 // N+3 .. 2N+2 the move indices by round, entries 2N+3 .. 3N+2 the bead each
 // move picked (by move index; the sweep then draws no pick block).  Records are interleaved by groups
 // of 8 slots (the chains of one sweep wavefront at 8 lanes per chain): entry
@@ -787,17 +788,18 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
   rec[sched_index(chain, stride, 0)] = (uint8_t)maxl;
   rec[sched_index(chain, stride, 1)] = 0;
   rec[sched_index(chain, stride, 2)] = 0;
+  // (only through the last round: the sweep reads round starts up to maxl + 1)
   int run = 0;
-  for (int l = 1; l <= length; ++l) {
+  for (int l = 1; l <= maxl; ++l) {
     int n = 0;
 #pragma unroll
     for (int j = 0; j <= CAPLOG; ++j) {
       const uint64_t w = NW == 1 || l <= 64 ? plane[j] : plane_hi[j];
       n |= (int)((w >> ((l - 1) & 63)) & 1u) << j;
     }
-    if (l <= maxl) at(W, l) = (uint8_t)run;
+    at(W, l) = (uint8_t)run;
     run += n;
-    rec[sched_index(chain, stride, 2 + l)] = (uint8_t)(l <= maxl ? run : 0);
+    rec[sched_index(chain, stride, 2 + l)] = (uint8_t)run;
   }
   for (int i = 0; i < length; ++i) {
     const int l = at(lev, i);
