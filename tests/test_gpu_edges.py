@@ -285,7 +285,7 @@ def test_schedule_kernel_long_chains(N, lanes):
     run_pair(spec, sweeps=2, seed=31, density_bits=32, lanes_per_chain=lanes)
 
 
-@pytest.mark.parametrize("lanes", [0, 4])
+@pytest.mark.parametrize("lanes", [0, 4, 8])
 def test_schedule_kernel_with_branched_chains(lanes):
     """66,000 chains (homopolymers, diblocks and 4-arm stars): enough chains
     that the move schedules come from the schedule kernels (register round
