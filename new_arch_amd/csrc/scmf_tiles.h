@@ -351,22 +351,29 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   for (int p = 0; p < kp.P; ++p)
     pop += (int64_t)(tile_start[p * g.ntiles + t + 1] - tile_start[p * g.ntiles + t]) * kp.len[p];
   // A tile of at most 65,535 beads cannot wrap a 16-bit window counter.  A
-  // larger tile is binned optimistically first: the atomics return the old
-  // words and every lane ORs them together.  A counter can only wrap after
-  // passing through 0x8000-0xFFFF, and some increment then sees bit 15 set
-  // (a carry out of a low half needs that half at 0xFFFF first), so if no
-  // lane saw bit 15 in either half the window is exact.  Otherwise the window
-  // is cleared and the tile binned again, normalising every NORM_BEADS beads
-  // so no half can wrap, with the beads outside the window (already counted
-  // globally) skipped.
+  // larger tile is binned optimistically first, and its window checked
+  // afterwards: every increment adds 1 to one uint16 half, a wrap of a half
+  // lowers the sum of all halves by at least 65535 (a carry out of a low half
+  // takes 65536 from it and adds 1 to the high half; a carry out of a word is
+  // lost), and nothing else lowers it, so the window is exact exactly when
+  // its halves sum to the number of beads counted in it (below 2^32).
+  // Otherwise the window is cleared and the tile binned again, normalising
+  // every NORM_BEADS beads so no half can wrap, with the beads outside the
+  // window (already counted globally) skipped.  (The check replaced atomics
+  // that returned the old words to be watched for bit 15: the returns made
+  // every pass wait for its atomics.)
   const bool checked = pop > 65535;
   __shared__ uint32_t redo_s;
   __shared__ uint32_t spare_s;  // target of the atomics of beads not counted in the window
   __shared__ uint32_t missed_s;  // beads of the tile counted with global atomics
+  __shared__ uint32_t here_s;    // beads counted in the window (checked tiles)
+  __shared__ uint32_t wsum_s;    // sum of the window's uint16 halves (checked tiles)
   if (threadIdx.x == 0) {
     redo_s = 0u;
     spare_s = 0u;
     missed_s = 0u;
+    here_s = 0u;
+    wsum_s = 0u;
   }
   const Axis ax0 = axis(kp, 0), ax1 = axis(kp, 1), ax2 = axis(kp, 2);
   // (kept in registers: the loop's global atomics would make the compiler
@@ -375,13 +382,13 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
   uint32_t missed = 0;
   for (int attempt = 0; attempt < 2; ++attempt) {
   const bool safe = attempt == 1;        // the normalising pass
-  const bool track = checked && !safe;   // returned words watched for bit 15
+  const bool track = checked && !safe;   // the window checked for wraps
   if (safe) {
     __syncthreads();
     if (redo_s == 0u) break;
     for (int w = threadIdx.x; w < g.win_words_padded; w += GRID_THREADS) win[w] = 0u;
   }
-  uint32_t seen = 0;
+  uint32_t nhere = 0;
   int since = 0;
   for (int p = 0; p < kp.P; ++p) {
     const int64_t s0 = tile_start[p * g.ntiles + t], s1 = tile_start[p * g.ntiles + t + 1];
@@ -400,8 +407,11 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
     // The type source is fixed per architecture; two copies of the loop keep
     // a global type load from sharing registers with the LDS one (which
     // would make every LDS read wait for the prefetched positions).
-    auto bin_range = [&](auto tab_c) {
+    // With CTY the lane's four monomer types are read once: the chain length
+    // divides the pass, so the lane's monomer indices never change.
+    auto bin_range = [&](auto tab_c, auto cty_c) {
       constexpr bool TAB = decltype(tab_c)::value;
+      constexpr bool CTY = decltype(cty_c)::value;
       // Each lane takes four beads per pass and keeps the loads of the next
       // PF passes in flight while it bins these (one pass a lane is ~4 MB in
       // flight on the whole GPU, ~16 KB per CU).  Loads are issued
@@ -420,6 +430,11 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
         m0 = (int)(rel - qd * len);
         if (m0 < 0) m0 += len;
         if (m0 >= len) m0 -= len;
+      }
+      uint32_t tyc[4] = {};
+      if constexpr (CTY) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) tyc[e] = (uint32_t)mono_s[m0 + e];
       }
       auto load = [&](int64_t gl, f32x4& Xl, f32x4& Yl, f32x4& Zl, u32v4& Ql) {
         const int64_t a = gl < b1 ? gl : last;
@@ -483,7 +498,9 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
             int xi, yi, zi;
             decode(e, &xi, &yi, &zi);
             sp[e] = tab_x[xi] + tab_y[yi] + tab_z[zi];
-            if (TAB)
+            if (CTY)
+              ty[e] = tyc[e];
+            else if (TAB)
               ty[e] = (uint32_t)mono_s[m0 + e];
             else
               ty[e] = e >= elo && e < ehi ? (uint32_t)btype[gq + e] * (PAIR ? 16u : 1u) : 0u;
@@ -494,7 +511,6 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
           // Every lane issues all four atomics: a bead that is not counted
           // here adds 0 to a spare word, so no branch (and no wait the
           // compiler would place before one) separates them.
-          uint32_t got[4];
 #pragma unroll
           for (int e = 0; e < 4; ++e) {
             const bool here = e >= elo && e < ehi && sp[e] < OUT_OF_WINDOW;
@@ -508,11 +524,9 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
               word = &win[wc >> 1];
               inc = 1u << ((wc & 1u) << 4);
             }
-            got[e] = atomicAdd(here ? word : &spare_s, here ? inc : 0u);
+            atomicAdd(here ? word : &spare_s, here ? inc : 0u);
+            nhere += here ? 1u : 0u;
           }
-          // (the returned words matter only when tracked)
-          const uint32_t got4 = (got[0] | got[1]) | (got[2] | got[3]);
-          seen |= track ? got4 : 0u;
           // beads outside the window (one wave-uniform test per pass)
           bool out[4];
 #pragma unroll
@@ -545,14 +559,27 @@ __global__ void __launch_bounds__(GRID_THREADS) k_grid_tiles(const KParams* __re
       }
       }
     };
-    if (tab)
-      bin_range(std::true_type{});
+    if (tab && PASS_BEADS % len == 0)
+      bin_range(std::true_type{}, std::true_type{});
+    else if (tab)
+      bin_range(std::true_type{}, std::false_type{});
     else
-      bin_range(std::false_type{});
+      bin_range(std::false_type{}, std::false_type{});
   }
   if (track) {
-    for (int off = 32; off > 0; off >>= 1) seen |= __shfl_xor(seen, off);
-    if ((threadIdx.x & 63) == 0 && (seen & 0x80008000u)) atomicOr(&redo_s, 1u);
+    for (int off = 32; off > 0; off >>= 1) nhere += __shfl_xor(nhere, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&here_s, nhere);
+    __syncthreads();
+    uint32_t sum = 0;
+    for (int w4 = threadIdx.x; 4 * w4 < g.win_words_padded; w4 += GRID_THREADS) {
+      const uint4 q = ((const uint4*)win)[w4];
+      sum += ((q.x & 0xFFFFu) + (q.x >> 16)) + ((q.y & 0xFFFFu) + (q.y >> 16)) +
+             ((q.z & 0xFFFFu) + (q.z >> 16)) + ((q.w & 0xFFFFu) + (q.w >> 16));
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(&wsum_s, sum);
+    __syncthreads();
+    if (threadIdx.x == 0 && wsum_s != here_s) redo_s = 1u;
   }
   }
   // beads counted globally: summed in LDS, one global atomic per tile (one

@@ -144,6 +144,32 @@ def test_crowded_cell(bits, crowd):
         g.close()
 
 
+@pytest.mark.parametrize("bits", [16, 32])
+def test_crowded_tile_without_wrap(bits):
+    """40,000 beads of type 0 and 30,000 of type 1 in one cell (both halves
+    of one window word): a tile of more than 65,535 beads whose halves pass
+    0x8000 but never wrap, so the window check passes without a second
+    binning.  Both count widths exact, no overflow."""
+    banner(f"crowded tile without wrap u{bits}")
+    a, b = W.homopolymer(1, 40_000, 0), W.homopolymer(1, 30_000, 1)
+    spec = SystemSpec(n_types=2, xn=W.xn_matrix(2, 40.0, 15.0), L=[2.0, 2.0, 2.0],
+                      n=[16, 16, 16], dtNxi=0.1, nref=16, polymers=[a, b])
+    g = scmf.System(spec, make_opts(seed=16, density_bits=bits, sort_every=1))
+    try:
+        o = orc.Oracle(spec, g.opts)
+        pos = o.init_coords()
+        pos[:] = np.array([0.51, 1.26, 0.77], np.float32)
+        g.set_positions(pos)
+        g.grid_density()
+        want = o.grid(pos)
+        assert want.max() == 40_000
+        assert np.array_equal(g.density(), want)
+        assert g.stats().overflow == 0
+        assert g.stats().sorts >= 1
+    finally:
+        g.close()
+
+
 @pytest.mark.parametrize("half", [0, 1])
 @pytest.mark.parametrize("n", [65_535, 65_536])
 def test_atomic_gridding_uint16_boundary(n, half):

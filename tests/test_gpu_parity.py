@@ -203,16 +203,20 @@ def tiled_spec(grid, T=2, N=16):
                       nref=N, polymers=poly)
 
 
+@pytest.mark.parametrize("grid,N", [((32, 32, 32), 16), ((32, 32, 32), 24), ((30, 32, 28), 16)])
 @pytest.mark.parametrize("cadence", [0, 1])
 @pytest.mark.parametrize("bits", [16, 32])
-def test_step_loop_on_ordered_two_type_system(bits, cadence):
+def test_step_loop_on_ordered_two_type_system(bits, cadence, grid, N):
     """scmf_step on a spatially ordered two-type system (the sweep clearing the
     counts, the tiled gridding advancing the stream word, orderings every 3
     sweeps): positions, counts and omega equal the oracle's after every step
     of both cadences (omega kept on steps 10 and 20 with the reference
-    cadence)."""
-    banner(f"ordered step loop u{bits} cadence={cadence}")
-    spec = tiled_spec((32, 32, 32), T=2)
+    cadence).  The gridding from cells takes computed window offsets on the
+    power-of-two grids (with each lane's types read once for N = 16, whose
+    length divides the 4096-bead pass, and per pass for N = 24) and the
+    axis tables on the 30 x 32 x 28 grid."""
+    banner(f"ordered step loop u{bits} cadence={cadence} grid {grid} N={N}")
+    spec = tiled_spec(grid, T=2, N=N)
     g, o = pair(spec, seed=41, density_bits=bits, sort_every=3, omega_every_step=bool(cadence))
     pos = o.init_coords()
     g.grid_density()
