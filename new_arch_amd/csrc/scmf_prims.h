@@ -38,14 +38,23 @@ struct u32x4 {
   uint32_t v[4];
 };
 
+// a ^ b ^ c: on gfx950 one v_bitop3_b32 (truth table 0x96), else two XORs
+SCMF_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 SCMF_HD void philox_round(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3, uint32_t k0,
                           uint32_t k1) {
   const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
   const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
   const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
   const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-  const uint32_t n0 = hi1 ^ c1 ^ k0;
-  const uint32_t n2 = hi0 ^ c3 ^ k1;
+  const uint32_t n0 = xor3(hi1, c1, k0);
+  const uint32_t n2 = xor3(hi0, c3, k1);
   c0 = n0;
   c1 = lo1;
   c2 = n2;

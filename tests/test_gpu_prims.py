@@ -3,6 +3,8 @@ oracle uses, bit for bit.  The device code differs (exp's Taylor steps take
 their coefficients from scalar registers; the Box-Muller radius uses a
 correctly rounded sqrt for its fixed argument range), so the results are
 pinned directly, beyond the trajectories of the parity tests:
+  * Philox4x32-10 blocks (the device rounds XOR three words in one
+    instruction) against the oracle's host build and a numpy restatement;
   * normal3f over every 24-bit uniform of both radii, so the fp32 log and
     the sqrt see each input they can ever get;
   * exp over wide, narrow, near-overflow, gradual-underflow and special
@@ -27,7 +29,40 @@ def check_lib():
     L = C.CDLL(LIB)
     L.scmf_check_normal3f.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
     L.scmf_check_exp.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+    L.scmf_check_philox.argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
     return L
+
+
+def philox_numpy(w):
+    """Philox4x32-10 (Salmon et al. 2011) of counter w[:, 0:4] and key
+    w[:, 4:6], vectorised: ten rounds of the two 32x32->64 products, the XOR
+    with the other counter words and the key, and the Weyl key schedule."""
+    c = [w[:, j].astype(np.uint64) for j in range(4)]
+    k0, k1 = w[:, 4].astype(np.uint64), w[:, 5].astype(np.uint64)
+    m32 = np.uint64(0xFFFFFFFF)
+    for _ in range(10):
+        p0 = np.uint64(0xD2511F53) * c[0]
+        p1 = np.uint64(0xCD9E8D57) * c[2]
+        c = [(p1 >> np.uint64(32)) ^ c[1] ^ k0, p1 & m32, (p0 >> np.uint64(32)) ^ c[3] ^ k1, p0 & m32]
+        k0 = (k0 + np.uint64(0x9E3779B9)) & m32
+        k1 = (k1 + np.uint64(0xBB67AE85)) & m32
+    return np.stack(c, 1).astype(np.uint32)
+
+
+def test_philox_device_equals_host():
+    print("[gpu-prims] philox", flush=True)
+    rng = np.random.default_rng(4)
+    n = 1 << 20
+    w = rng.integers(0, 2 ** 32, (n, 6), dtype=np.uint64).astype(np.uint32)
+    w[:4] = 0
+    w[4:8] = np.uint32(0xFFFFFFFF)
+    dev = np.empty((n, 4), np.uint32)
+    assert check_lib().scmf_check_philox(w.ctypes.data, n, dev.ctypes.data) == 0
+    want = philox_numpy(w)
+    bad = np.nonzero((dev != want).any(1))[0]
+    assert bad.size == 0, f"{bad.size} mismatches, first block {bad[0]}: {w[bad[0]]}"
+    for i in list(range(8)) + [12345, n - 1]:
+        assert np.array_equal(dev[i], orc.philox(w[i, :4], w[i, 4:]))
 
 
 def test_normal3f_every_radius_uniform():
