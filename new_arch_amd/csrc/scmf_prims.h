@@ -296,23 +296,15 @@ SCMF_HD void sincosf_turn24(uint32_t k, float* s_out, float* c_out) {
   cp = __builtin_fmaf(cp, t2, 1.0f / 24.0f);
   cp = __builtin_fmaf(cp, t2, -0.5f);
   const float cs = __builtin_fmaf(t2, cp, 1.0f);
+  // quarter turn q: odd quadrants swap sin and cos; sin is negated in
+  // quadrants 2 and 3, cos in 1 and 2 (sign-bit flips, as negation is), so
+  // the selection needs no branch: (sn, cs), (cs, -sn), (-sn, -cs), (-cs, sn)
   const uint32_t quad = q & 3u;
-  float s, c;
-  if (quad == 0) {
-    s = sn;
-    c = cs;
-  } else if (quad == 1) {
-    s = cs;
-    c = -sn;
-  } else if (quad == 2) {
-    s = -sn;
-    c = -cs;
-  } else {
-    s = -cs;
-    c = sn;
-  }
-  *s_out = s;
-  *c_out = c;
+  const float a = (quad & 1u) ? cs : sn;
+  const float b = (quad & 1u) ? sn : cs;
+  const uint32_t sneg = (quad & 2u) << 30, cneg = ((quad + 1u) & 2u) << 30;
+  *s_out = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, a) ^ sneg);
+  *c_out = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, b) ^ cneg);
 }
 
 // Correctly rounded sqrt of x = -2 log(u), u a 24-bit uniform in (0, 1]: x
