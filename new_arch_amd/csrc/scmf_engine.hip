@@ -730,29 +730,19 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
   uint8_t* const lev = W + (nmax + 2) * SCHED_LANES;    // [nmax] round of each move
   auto at = [](uint8_t* a, int k) -> uint8_t& { return a[k * SCHED_LANES]; };
   SCMF_GLOBAL uint8_t* __restrict__ rec = gptr(kp.sched);
-  u32x4 blk{};
-  auto pick = [&](int i) {  // one pick block per two moves
-    if ((i & 1) == 0) blk = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i >> 1, SLOT_PICK);
-    return pick_bead(pick_u01(blk, (uint32_t)i), length);
-  };
   for (int k = 0; k <= length + 1; ++k) at(W, k) = 0;
-  // rounds, the next move's pick drawn while this move's LDS reads are in flight
   const bool linear = kp.linear[p] != 0;
   int maxl = 0;
-  int b = length > 0 ? pick(0) : 0;
   uint64_t plane[CAPLOG + 1] = {}, plane_hi[CAPLOG + 1] = {};  // plane_hi: rounds 65 .. 128
-  for (int i = 0; i < length; ++i) {
-    const int cur = b;
+  // the round of move i on bead cur
+  auto step = [&](int i, int cur) {
     int l;
     if (linear) {
-      const int w0 = at(W, cur), w1 = at(W, cur + 1), w2 = at(W, cur + 2);
-      if (i + 1 < length) b = pick(i + 1);
-      l = max(max(w0, w1), w2) + 1;
+      l = max(max((int)at(W, cur), (int)at(W, cur + 1)), (int)at(W, cur + 2)) + 1;
     } else {
       l = at(W, cur + 1);
       for_neighbours(kp, p, cur, length, [&](int j) { l = max(l, (int)at(W, j + 1)); });
       l += 1;
-      if (i + 1 < length) b = pick(i + 1);
     }
     uint64_t carry, carry_hi = 0;
     if constexpr (NW == 1) {
@@ -782,6 +772,15 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
     at(lev, i) = (uint8_t)l;
     rec[sched_index(chain, stride, 2 * length + 3 + i)] = (uint8_t)cur;
     maxl = max(maxl, l);
+  };
+  // rounds: one pick block per two moves, drawn at the top of each pair
+  // (a block drawn under a condition carried across iterations kept the
+  // key schedule in vector registers)
+  for (int i = 0; i < length; i += 2) {
+    const u32x4 blk = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i >> 1, SLOT_PICK);
+    const int b1 = pick_bead(pick_u01(blk, (uint32_t)i + 1), length);
+    step(i, pick_bead(pick_u01(blk, (uint32_t)i), length));
+    if (i + 1 < length) step(i + 1, b1);
   }
   // round starts (start[0] = start[1] = 0, prefix sums through the last
   // round, zero after it); W[l] becomes round l's next list position
