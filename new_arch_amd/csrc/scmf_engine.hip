@@ -734,6 +734,11 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
   const bool linear = kp.linear[p] != 0;
   int maxl = 0;
   uint64_t plane[CAPLOG + 1] = {}, plane_hi[CAPLOG + 1] = {};  // plane_hi: rounds 65 .. 128
+  const double n53 = (double)length * 0x1p-53;
+  // the record's regions (entry k + 1 lies 8 bytes after entry k)
+  SCMF_GLOBAL uint8_t* const bead_rec = rec + sched_index(chain, stride, 2 * length + 3);
+  SCMF_GLOBAL uint8_t* const list_rec = rec + sched_index(chain, stride, length + 3);
+  SCMF_GLOBAL uint8_t* const start_rec = rec + sched_index(chain, stride, 2);
   // the round of move i on bead cur
   auto step = [&](int i, int cur) {
     int l;
@@ -770,7 +775,7 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
     }
     at(W, cur + 1) = (uint8_t)l;
     at(lev, i) = (uint8_t)l;
-    rec[sched_index(chain, stride, 2 * length + 3 + i)] = (uint8_t)cur;
+    bead_rec[8 * i] = (uint8_t)cur;
     maxl = max(maxl, l);
   };
   // rounds: one pick block per two moves, drawn at the top of each pair
@@ -778,8 +783,8 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
   // key schedule in vector registers)
   for (int i = 0; i < length; i += 2) {
     const u32x4 blk = stream_block(kp.seed, kp.system_id, id, sweep, (uint32_t)i >> 1, SLOT_PICK);
-    const int b1 = pick_bead(pick_u01(blk, (uint32_t)i + 1), length);
-    step(i, pick_bead(pick_u01(blk, (uint32_t)i), length));
+    const int b1 = pick_of(blk, (uint32_t)i + 1, length, n53);
+    step(i, pick_of(blk, (uint32_t)i, length, n53));
     if (i + 1 < length) step(i + 1, b1);
   }
   // round starts (start[0] = start[1] = 0, prefix sums through the last
@@ -798,13 +803,13 @@ __global__ void __launch_bounds__(SCHED_LANES) k_schedule_reg(const KParams* __r
     }
     at(W, l) = (uint8_t)run;
     run += n;
-    rec[sched_index(chain, stride, 2 + l)] = (uint8_t)run;
+    start_rec[8 * l] = (uint8_t)run;
   }
   for (int i = 0; i < length; ++i) {
     const int l = at(lev, i);
     const int pos = at(W, l);
     at(W, l) = (uint8_t)(pos + 1);
-    rec[sched_index(chain, stride, length + 3 + pos)] = (uint8_t)i;
+    list_rec[8 * pos] = (uint8_t)i;
   }
 }
 

@@ -125,6 +125,17 @@ SCMF_HD int pick_bead(double u, int n) {
   const int b = (int)(u * n);
   return b == n ? n - 1 : b;
 }
+// pick_bead(pick_u01(w, i), n) with n53 = n 2^-53, in fewer operations: u
+// is m 2^-53 for the 53-bit integer m = hi 2^26 + lo + 1 (one exact fma of
+// the two exactly converted halves), and m n53 is the real number u n, so
+// its single rounding is that of u * n.
+SCMF_HD int pick_of(const u32x4& w, uint32_t i, int n, double n53) {
+  const uint32_t a = (i & 1u) ? w.v[2] : w.v[0];
+  const uint32_t b = (i & 1u) ? w.v[3] : w.v[1];
+  const double m = __builtin_fma((double)(a >> 5), 67108864.0, (double)((b >> 6) + 1u));
+  const int k = (int)(m * n53);
+  return k == n ? n - 1 : k;
+}
 // The accept uniform of a move, (0, 1] with 24 bits, from the low bytes of
 // words 0, 1, 2 of its SLOT_DISP block.  Exact.
 SCMF_HD float accept_u01(const u32x4& disp) {

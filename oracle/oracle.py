@@ -51,6 +51,8 @@ def lib():
     L.orc_re2.restype = C.c_double
     L.orc_philox.argtypes = [C.c_uint32] * 6 + [vp]
     L.orc_philox.restype = None
+    L.orc_pick_disagreements.argtypes = [vp, C.c_int64, C.c_int32]
+    L.orc_pick_disagreements.restype = C.c_int64
     for n in ("orc_exp_many", "orc_log_many"):
         getattr(L, n).argtypes = [vp, vp, C.c_int64]
         getattr(L, n).restype = None
@@ -143,6 +145,13 @@ class Oracle:
         out = np.empty(t.size, np.uint32)
         lib().orc_bin_many(C.byref(self.desc), _ptr(r), t.size, _ptr(t), _ptr(out))
         return out
+
+
+def pick_disagreements(words, n):
+    """Picks (moves 0 and 1 of each 4-word block) on which the engine's
+    pick_of and the defining pick_bead(pick_u01(.)) differ, for chains of n beads."""
+    w = np.ascontiguousarray(words, np.uint32).reshape(-1, 4)
+    return int(lib().orc_pick_disagreements(_ptr(w), w.shape[0], int(n)))
 
 
 def philox(ctr, key):

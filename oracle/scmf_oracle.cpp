@@ -527,6 +527,20 @@ void orc_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
   const u32x4 r = philox4x32_10(c0, c1, c2, c3, k0, k1);
   for (int i = 0; i < 4; ++i) out[i] = r.v[i];
 }
+// Blocks of w (four words each) whose picks of moves 0 and 1 on chains of n
+// beads differ between the defining pick_bead(pick_u01(.)) and the engine's
+// pick_of (the count of such picks; zero when the two forms agree).
+int64_t orc_pick_disagreements(const uint32_t* w, int64_t blocks, int32_t n) {
+  const double n53 = (double)n * 0x1p-53;
+  int64_t bad = 0;
+  for (int64_t k = 0; k < blocks; ++k) {
+    u32x4 b;
+    for (int j = 0; j < 4; ++j) b.v[j] = w[4 * k + j];
+    for (uint32_t i = 0; i < 2; ++i)
+      bad += pick_of(b, i, n, n53) != pick_bead(pick_u01(b, i), n) ? 1 : 0;
+  }
+  return bad;
+}
 void orc_exp_many(const double* x, double* y, int64_t n) {
   for (int64_t i = 0; i < n; ++i) y[i] = exp_cw(x[i]);
 }

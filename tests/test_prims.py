@@ -134,3 +134,22 @@ def test_streams_differ_by_slot(key):
     a = orc.philox([5, 0, (3 << 2) | 0, key[1]], [key[0], 0])
     b = orc.philox([5, 0, (3 << 2) | 1, key[1]], [key[0], 0])
     assert not np.array_equal(a, b)
+
+
+def test_pick_forms_agree():
+    """The engine's pick (the 53-bit uniform as one exact fma of its halves,
+    times n 2^-53) equals the defining (int)(u n) with the last bead for
+    u = 1, on random blocks and on the words at the ends of both halves,
+    for chain lengths up to past 255."""
+    rng = np.random.default_rng(17)
+    w = rng.integers(0, 2 ** 32, (400_000, 4), dtype=np.uint64).astype(np.uint32)
+    edge = np.array([0, 1, 31, 32, 63, 64, 2 ** 31, 2 ** 32 - 65, 2 ** 32 - 64, 2 ** 32 - 1],
+                    np.uint32)
+    a, b = np.meshgrid(edge, edge)
+    e = np.stack([a.ravel(), b.ravel(), a.ravel(), b.ravel()], 1).astype(np.uint32)
+    words = np.concatenate([w, e])
+    for n in (1, 2, 3, 7, 31, 32, 63, 64, 65, 96, 127, 128, 160, 255, 300, 4500, 1 << 20):
+        assert orc.pick_disagreements(words, n) == 0, n
+    # a uniform of exactly 1 (both halves all ones) picks the last bead
+    top = np.array([[2 ** 32 - 1] * 4], np.uint32)
+    assert orc.pick_disagreements(top, 64) == 0
