@@ -8,7 +8,7 @@ uint16 counts, fp32 coordinates; synthetic Gaussian-walk initial state.
 
 Prints one JSON line.  `value` is bead-moves per second over the timed steps
 (positions, counts and fields resident in HBM throughout); the timed window
-starts at a spatial ordering (every 25 steps) and holds ceil(steps / 25) of
+starts at a spatial ordering (every 100 steps) and holds ceil(steps / 100) of
 them.  `roofline` is for the dominant kernel (the sweep: move schedules +
 moves), from HIP events on the engine's stream, against the scope table's
 algorithmic bytes, with the counter-based VALU issue figure beside it;
@@ -375,10 +375,14 @@ def issue_figures(workload, kernels, B):
             "source": f"profiles/{workload}_valu.json, profiles/valu_calibration.json"}
 
 
+# the engine's automatic spatial-ordering period (systems of 2^23 beads or more)
+ORDER_PERIOD = 100
+
+
 def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True):
     """Create the named system, autotune the sweep shape, run `warmup` steps
     (plus the untimed steps up to the next spatial ordering, so the timed
-    window starts with one and holds ceil(steps / 25) of them), then time
+    window starts with one and holds ceil(steps / ORDER_PERIOD) of them), then time
     `steps` steps (sweep + gridding + omega every sweep)."""
     from new_arch_amd import scmf
     from new_arch_amd import workloads as W
@@ -397,7 +401,7 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
     log(f"created in {time.perf_counter() - t:.1f} s")
     lanes, tuned_ms = g.autotune(2) if autotune else (0, 0.0)
     log(f"autotune: {lanes} lanes per chain, {tuned_ms:.3f} ms per sweep")
-    period = 25 if spec.n_beads >= (1 << 23) else 0   # automatic ordering period
+    period = ORDER_PERIOD if spec.n_beads >= (1 << 23) else 0   # automatic ordering period
     step_no = 0
     for _ in range(warmup):
         step_no += 1
@@ -636,7 +640,7 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=25)  # one spatial ordering period
+    ap.add_argument("--steps", type=int, default=ORDER_PERIOD)  # one spatial ordering period
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="large_diblock_melt")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -684,7 +688,7 @@ def main():
         out["small"] = sm
     if not args.no_largest and args.workload != "largest_blend":
         log("largest system")
-        lg = measure_system("largest_blend", steps=25, warmup=1)
+        lg = measure_system("largest_blend", steps=ORDER_PERIOD, warmup=1)
         lg.pop("_spec"), lg.pop("_opts")
         out["largest"] = lg
     if not args.no_batched:

@@ -77,8 +77,9 @@ typedef struct scmf_opts {
                                memory; 0 = pick automatically.  Results do not
                                depend on it. */
   int32_t sort_every;       /* reorder chains spatially every k sweeps (enables the
-                               tiled gridding); 0 = automatic, -1 = never.
-                               Results do not depend on it. */
+                               tiled gridding); 0 = automatic (every 100 sweeps for
+                               systems of 2^23 beads or more, else never), -1 =
+                               never.  Results do not depend on it. */
 } scmf_opts;
 
 typedef struct scmf_stats_t {
