@@ -507,6 +507,28 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
     off = kp.bead_base[p] + (chain - kp.chain_base[p]) * length;
     id = gptr((const uint32_t*)kp.slot_chain)[chain];
   }
+  // the wave's eight chains are one record group (records interleave groups
+  // of 8 slots), all nmax beads long: their entries load 8 bytes at a time
+  bool group = false;
+  if constexpr (L == 8)
+    group = ((c0 + (int64_t)br.local * G) & 7) == 0 && __all(active && length == nmax);
+  // For records of at most 256 entries the group's loads are issued before
+  // the tile's, so both wait together (four 8-byte entries a lane).
+  const int st = kp.sched_stride;
+  const bool early = pre && group && 3 * nmax + 2 <= 256;
+  uint64_t ge[4] = {};
+  int maxl = 0;
+  const SCMF_GLOBAL uint64_t* grp = nullptr;
+  if (pre && group)
+    grp = (const SCMF_GLOBAL uint64_t*)(sched + ((c0 + (int64_t)br.local * G) >> 3) * (int64_t)st * 8);
+  if (early) {
+    if (active) maxl = sched[sched_index(chain, st, 0)];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = 1 + lane + 64 * j;
+      if (k <= 3 * nmax + 2) ge[j] = grp[k];
+    }
+  }
   // load the tile, eight beads a lane in flight at once (the loop with one
   // bead an iteration waited for each: 0.18 ms more a sweep at 268M beads)
   for (int b0 = r; b0 < length; b0 += 8 * L) {
@@ -530,23 +552,23 @@ __global__ void __launch_bounds__(64, 5) k_sweep_sched(const KParams* __restrict
       }
     }
   }
-  // the wave's eight chains are one record group (records interleave groups
-  // of 8 slots), all nmax beads long: their entries load 8 bytes at a time
-  bool group = false;
-  if constexpr (L == 8)
-    group = ((c0 + (int64_t)br.local * G) & 7) == 0 && __all(active && length == nmax);
-  int maxl = 0;
   if (pre) {
-    const int st = kp.sched_stride;
-    if (group) {
+    if (early) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = 1 + lane + 64 * j;
+        if (k <= 3 * nmax + 2) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) book[q * per + k - 1] = (I)((ge[j] >> (8 * q)) & 0xFFu);
+        }
+      }
+    } else if (group) {
       // the group's entries 1 .. 3N+2 (start, list, bead: book offsets 0 ..
       // 3N+1 of each slot), one 8-byte load per entry for all eight slots
       // (0.15 ms less a sweep at 268M beads than a byte load per lane and
       // entry; the same treatment of the tile's loads gained nothing, and of
       // its write-back and cells cost 0.35 ms)
       if (active) maxl = sched[sched_index(chain, st, 0)];
-      const SCMF_GLOBAL uint64_t* grp = (const SCMF_GLOBAL uint64_t*)(
-          sched + ((c0 + (int64_t)br.local * G) >> 3) * (int64_t)st * 8);
       for (int k = 1 + threadIdx.x; k <= 3 * nmax + 2; k += 64) {
         const uint64_t e = grp[k];
 #pragma unroll
