@@ -455,9 +455,9 @@ __global__ void __launch_bounds__(64) k_sweep_lds(const KParams* __restrict__ Ps
 // around, then per-round fill counts); start[N+2] (moves per round during
 // the scan, then the first list entry of each round); list[N] (moves by
 // round); rnd[N] (the round of each move).  With a schedule made by
-// the schedule kernels only start and list are kept.
-// (At most 96 VGPRs, i.e. five waves per SIMD: the LDS tiles allow about
-// as many, and the move is issue-bound, so occupancy is worth the spills.)
+// the schedule kernels the book keeps start, list and the bead of each move.
+// (Five waves per SIMD, at most 102 VGPRs: the LDS tiles allow about as
+// many, and the move is issue-bound, so occupancy is worth registers.)
 // Byte k of the move-schedule record of a slot (records interleaved by
 // groups of 8 slots; made by k_schedule_reg or k_schedule_lds below).
 __device__ __forceinline__ int64_t sched_index(int64_t slot, int stride, int k) {
