@@ -716,7 +716,7 @@ size_t schedule_record_bytes(int nmax, int64_t chains) {
 }
 // Whether the register-count kernel takes chains of nmax beads in rounds of cap moves.
 bool schedule_in_registers(int nmax, int cap) {
-  return nmax <= 128 && (cap == 2 || cap == 4 || cap == 8);
+  return nmax <= 128 && (cap == 2 || cap == 4 || cap == 8 || cap == 16);
 }
 size_t schedule_lds_bytes(int nmax, int cap) {
   return (size_t)(schedule_in_registers(nmax, cap) ? 2 * nmax + 2 : 4 * nmax + 5) * SCHED_LANES;
@@ -1421,6 +1421,7 @@ struct scmf_sys {
   uint32_t* d_ctr = nullptr;
   size_t lds_bytes = 0; // LDS of the sweep kernel
   const SweepVariant* variant = nullptr;
+  int lds_per_cu = 0, lds_per_block = 0;  // the device's LDS limits
   float* x = nullptr;
   float* y = nullptr;
   float* z = nullptr;
@@ -1656,6 +1657,8 @@ int choose_sweep(scmf_sys* s) {
   (void)hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor,
                               s->device);
   if (per_cu <= 0) per_cu = max_block;
+  s->lds_per_cu = per_cu;
+  s->lds_per_block = max_block;
   const bool wide = s->nmax > 255;
   const SweepVariant* best = &kVariants[0];
   int best_waves = 0;
@@ -1711,7 +1714,8 @@ int launch_schedule(scmf_sys* s, int cap, int64_t c0, int64_t c1, int nmax) {
     const bool two = nmax > 64;
     const void* fn = cap == 2 ? (two ? (const void*)k_schedule_reg<1, 2> : (const void*)k_schedule_reg<1, 1>)
                    : cap == 4 ? (two ? (const void*)k_schedule_reg<2, 2> : (const void*)k_schedule_reg<2, 1>)
-                              : (two ? (const void*)k_schedule_reg<3, 2> : (const void*)k_schedule_reg<3, 1>);
+                   : cap == 8 ? (two ? (const void*)k_schedule_reg<3, 2> : (const void*)k_schedule_reg<3, 1>)
+                              : (two ? (const void*)k_schedule_reg<4, 2> : (const void*)k_schedule_reg<4, 1>);
     void* args[] = {(void*)&kp, (void*)&nmax, (void*)&c0, (void*)&c1};
     HIP_TRY(hipLaunchKernel(fn, grid, dim3(SCHED_LANES), args, lds, s->stream));
   } else {
@@ -1723,6 +1727,23 @@ int launch_schedule(scmf_sys* s, int cap, int64_t c0, int64_t c1, int nmax) {
                             s->stream));
   }
   return SCMF_OK;
+}
+
+// The variant that sweeps a run of chains of at most nmax beads: the chosen
+// one, except that at 8 lanes a chain a run of chains longer than 64 beads
+// (whose LDS tiles would leave fewer waves resident) takes 16 lanes a chain
+// with schedules in rounds of 16 when that keeps more waves resident (the
+// 128-bead run of the 1.0e9-bead blend: 2.5 waves a SIMD at 8 lanes, 5 at 16).
+const SweepVariant* run_variant(const scmf_sys* s, const SweepVariant* v, int nmax) {
+  if (v->L != 8 || !s->pre_sched || nmax <= 64 || s->lds_per_cu <= 0) return v;
+  const SweepVariant* v16 = nullptr;
+  for (const SweepVariant& w : kVariants)
+    if (w.L == 16 && w.wide == v->wide) v16 = &w;
+  if (!v16) return v;
+  auto resident = [&](int L) {  // waves per CU that LDS allows, at most the 20 registers allow
+    return std::min(20, (int)((size_t)s->lds_per_cu / sched_pre_lds_bytes(L, nmax)));
+  };
+  return resident(16) > resident(8) && sched_pre_lds_bytes(16, nmax) <= (size_t)s->lds_per_block ? v16 : v;
 }
 
 // clear: the sweep also clears the counts (scmf_step, whose gridding follows);
@@ -1740,29 +1761,37 @@ int launch_sweep(scmf_sys* s, bool clear = false, bool advance = true) {
     hipLaunchKernelGGL(k_sweep_global, dim3(grid_for(s->n_chains)), dim3(BLOCK), 0, s->stream,
                        kpp, none, clr, c0, c1);
   } else {
-    const int G = 64 / v->L;
+    // each run's variant (and so its schedules' round size)
+    std::vector<const SweepVariant*> vr(s->runs.size(), v);
     if (s->pre_sched) {
       bool split = true;
-      for (const auto& r : s->runs) split = split && schedule_in_registers(r.nmax, v->L);
+      for (size_t k = 0; k < s->runs.size(); ++k) {
+        vr[k] = run_variant(s, v, s->runs[k].nmax);
+        split = split && schedule_in_registers(s->runs[k].nmax, vr[k]->L);
+      }
       if (split) {
-        for (const auto& r : s->runs) {
-          const int rc = launch_schedule(s, v->L, r.c0, r.c1, r.nmax);
+        for (size_t k = 0; k < s->runs.size(); ++k) {
+          const auto& r = s->runs[k];
+          const int rc = launch_schedule(s, vr[k]->L, r.c0, r.c1, r.nmax);
           if (rc != SCMF_OK) return rc;
         }
-      } else {
+      } else {  // one schedule launch over every chain: one round size for all runs
+        std::fill(vr.begin(), vr.end(), v);
         const int rc = launch_schedule(s, v->L, 0, s->n_chains, s->nmax);
         if (rc != SCMF_OK) return rc;
       }
     }
     for (size_t k = 0; k < s->runs.size(); ++k) {
+      const SweepVariant* w = vr[k];
+      const int G = 64 / w->L;
       int nm = s->runs[k].nmax;
       int clr = clear && k == 0 ? 1 : 0;  // the first launch clears every count
       int64_t c0 = s->runs[k].c0, c1 = s->runs[k].c1;
-      const size_t lds = s->pre_sched && v->L >= 2 ? sched_pre_lds_bytes(v->L, nm) : v->bytes(nm);
-      if (lds > 65536) HIP_TRY(raise_lds_limit(v->fn, lds));
+      const size_t lds = s->pre_sched && w->L >= 2 ? sched_pre_lds_bytes(w->L, nm) : w->bytes(nm);
+      if (lds > 65536) HIP_TRY(raise_lds_limit(w->fn, lds));
       const dim3 grid((unsigned)((c1 - c0 + G - 1) / G));
       void* args[] = {(void*)&kpp, (void*)&none, (void*)&nm, (void*)&clr, (void*)&c0, (void*)&c1};
-      HIP_TRY(hipLaunchKernel(v->fn, grid, dim3(64), args, lds, s->stream));
+      HIP_TRY(hipLaunchKernel(w->fn, grid, dim3(64), args, lds, s->stream));
     }
   }
   if (advance)

@@ -299,19 +299,22 @@ def test_gridding_follows_every_position_change(tmp_path, L, lanes):
         g.close()
 
 
-@pytest.mark.parametrize("N,lanes", [(96, 2), (96, 8), (96, 16), (128, 2), (160, 2), (160, 8)])
+@pytest.mark.parametrize("N,lanes", [(96, 2), (96, 8), (96, 16), (128, 2), (128, 8), (160, 2),
+                                     (160, 8)])
 def test_schedule_kernel_long_chains(N, lanes):
-    """70,000 diblocks of N beads, rounds of 2 or 8 moves.  Chains of 65-128
+    """70,000 diblocks of N beads, rounds of 2, 8 or 16 moves.  Chains of 65-128
     beads take the register-count schedule kernel with two 64-bit words per
-    bit-plane (rounds of 2 moves reach past round 64 there); longer chains take
-    the kernel that counts each round's moves in LDS (k_schedule_lds, records
-    staged in LDS).  Sweeps bit-exact against the oracle."""
+    bit-plane (rounds of 2 moves reach past round 64 there); at 8 lanes a
+    chain they are swept at 16 lanes (rounds of 16 moves, five bit-planes),
+    which keeps more waves resident; longer chains take the kernel that counts
+    each round's moves in LDS (k_schedule_lds, records staged in LDS).  Sweeps
+    bit-exact against the oracle."""
     banner(f"schedule kernel, N={N}, lanes {lanes}")
     spec = W.tiny(n_chains=70_000, N=N, grid=(32, 32, 32), L=(16.0, 16.0, 16.0))
     run_pair(spec, sweeps=2, seed=31, density_bits=32, lanes_per_chain=lanes)
 
 
-@pytest.mark.parametrize("lanes", [0, 4, 8])
+@pytest.mark.parametrize("lanes", [0, 4, 8, 16])
 def test_schedule_kernel_with_branched_chains(lanes):
     """66,000 chains (homopolymers, diblocks and 4-arm stars): enough chains
     that the move schedules come from the schedule kernels (register round
