@@ -377,6 +377,26 @@ def issue_figures(workload, kernels, B):
 
 # the engine's automatic spatial-ordering period (systems of 2^23 beads or more)
 ORDER_PERIOD = 100
+LDS_PER_CU = 160 * 1024   # MI355X LDS per compute unit
+
+
+def sweep_run_lanes(spec, lanes, pre):
+    """(chain length, lanes a chain) of each sweep launch, as the engine makes
+    them: consecutive architectures of one length share a launch, and at 8
+    lanes a chain with pre-made schedules a run of chains longer than 64 beads
+    takes 16 lanes when LDS then keeps more waves resident (at most 20 a CU)."""
+    lengths = []
+    for p in spec.polymers:
+        if p.count > 0 and (not lengths or lengths[-1] != p.length):
+            lengths.append(p.length)
+    if spec.n_chains < 65536:
+        lengths = [max(lengths)]
+
+    def resident(L, n):
+        G = 64 // L
+        return min(20, LDS_PER_CU // (3 * n * G * 4 + G * (3 * n + 4)))
+    return [(n, 16 if lanes == 8 and pre and n > 64 and resident(16, n) > resident(8, n) else lanes)
+            for n in lengths]
 
 
 def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True):
@@ -545,17 +565,26 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
     omega_bytes = s * TC + 4 * TC       # counts read, fp32 omega written
     tiled = spec.n_beads >= (1 << 23)
     sweep_kernel = {-1: "k_sweep_global", 1: "k_sweep_lds"}.get(lanes, f"k_sweep_sched<{lanes}>")
+    if lanes > 1:
+        sweep_kernel = "+".join(f"k_sweep_sched<{L}>"
+                                for L in sorted({L for _, L in sweep_run_lanes(spec, lanes, pre)}))
 
     # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE runs
     # of this workload (calibration and commands in the profile summary).
     measured = profile_json(workload, "traffic").get("bytes_per_launch", {})
     byte_t = "unsigned char" if nmax <= 255 else "unsigned short"
+    run_lanes = sweep_run_lanes(spec, lanes, pre)
     # the schedule kernel keeps round counts in registers for chains of <= 128 beads
-    # and rounds of 2, 4 or 8 moves
-    sched_kernel = (f"k_schedule_reg<{ {2: 1, 4: 2, 8: 3}[lanes]}, {1 if nmax <= 64 else 2}>"
-                    if nmax <= 128 and lanes in (2, 4, 8) else "k_schedule_lds")
-    sweep_keys = ([f"k_sweep_sched<{lanes}, {byte_t}>"] if lanes > 1 else [sweep_kernel]) + \
-        ([sched_kernel] if pre else [])
+    # and rounds of 2, 4, 8 or 16 moves (else one k_schedule_lds launch at the
+    # sweep's lanes over every chain)
+    in_regs = all(n <= 128 for n, _ in run_lanes)
+    if not in_regs:
+        run_lanes = [(n, lanes) for n, _ in run_lanes]
+    sched_keys = ([f"k_schedule_reg<{c}, {w}>" for c, w in
+                   sorted({(L.bit_length() - 1, 1 if n <= 64 else 2) for n, L in run_lanes})]
+                  if in_regs else ["k_schedule_lds"])
+    sweep_keys = ([f"k_sweep_sched<{L}, {byte_t}>" for L in sorted({L for _, L in run_lanes})]
+                  if lanes > 1 else [sweep_kernel]) + (sched_keys if pre else [])
     omega_key = (f"k_omega4<{spec.n_types}, {bits}>" if (spec.n_cells % 4 == 0)
                  else "k_omega")
     pair = "true" if spec.n_types == 2 else "false"
@@ -601,7 +630,7 @@ def measure_system(workload, steps, warmup, with_host_copy=False, autotune=True)
     dtNxi, nref = spec.dtNxi, spec.nref
     A = dtNxi / nref
     roofline = dict(roof(sweep_bytes, tm.sweep_ms, traffic_of(sweep_keys)),
-                    kernel=(sched_kernel + "+" if pre else "") + sweep_kernel,
+                    kernel=("+".join(sched_keys) + "+" if pre else "") + sweep_kernel,
                     implementation_bytes=impl)
     issue = issue_figures(workload, sweep_keys, B)
     if issue:
