@@ -74,8 +74,10 @@ typedef struct scmf_opts {
                                0: the caller supplies positions (scmf_copy_in_pos) */
   int32_t lanes_per_chain;  /* sweep kernel shape: 1, 2, 4, 8 or 16 lanes share a chain
                                staged in LDS; -1 = one lane per chain in global
-                               memory; 0 = pick automatically.  Results do not
-                               depend on it. */
+                               memory; 0 = pick automatically.  At 8, systems of
+                               65,536 chains or more sweep their runs of chains
+                               longer than 64 beads at 16 when that keeps more
+                               waves resident.  Results do not depend on it. */
   int32_t sort_every;       /* reorder chains spatially every k sweeps (enables the
                                tiled gridding); 0 = automatic (every 100 sweeps for
                                systems of 2^23 beads or more, else never), -1 =

@@ -314,6 +314,21 @@ def test_schedule_kernel_long_chains(N, lanes):
     run_pair(spec, sweeps=2, seed=31, density_bits=32, lanes_per_chain=lanes)
 
 
+@pytest.mark.parametrize("long_first", [False, True])
+def test_blend_runs_at_their_own_lanes(long_first):
+    """A blend of 64-bead homopolymers and 128-bead diblocks (70,000 chains) at
+    8 lanes a chain: the 64-bead run sweeps at 8 lanes, the 128-bead run at 16
+    (rounds of 16 moves, its own schedule launch), in either order of the runs.
+    Sweeps bit-exact against the oracle."""
+    banner(f"blend runs, 128-bead run first: {long_first}")
+    homo = [W.homopolymer(64, 35_000, 0), W.homopolymer(64, 25_000, 1)]
+    dib = [W.diblock(128, 0.5, 10_000)]
+    spec = SystemSpec(n_types=2, xn=W.xn_matrix(2, 50.0, 15.0), L=[16.0, 16.0, 16.0],
+                      n=[32, 32, 32], dtNxi=0.05, nref=64,
+                      polymers=dib + homo if long_first else homo + dib)
+    run_pair(spec, sweeps=2, seed=37, density_bits=32, lanes_per_chain=8, move="smart")
+
+
 @pytest.mark.parametrize("lanes", [0, 4, 8, 16])
 def test_schedule_kernel_with_branched_chains(lanes):
     """66,000 chains (homopolymers, diblocks and 4-arm stars): enough chains
